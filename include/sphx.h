@@ -1,0 +1,247 @@
+/*
+ * sphx.h -- C ABI of the MI355X-native weakly-compressible SPH step.
+ *
+ * Every entry point below replaces one computing-kernel (CK) dynamics class of
+ * SPHinXsys when the case file selects the HIP execution policy instead of
+ * par_host.  The reference class each call stands in for is cited as
+ * "reference:" with file:line relative to the SPHinXsys source tree.
+ *
+ * Conventions
+ *  - All functions return sphx_status (0 = SPHX_OK).  On failure the message
+ *    is available from sphx_last_error() (thread-local).  Nothing calls exit().
+ *  - Host arrays are borrowed only for the duration of a call.
+ *  - Vectors cross the boundary as AoS doubles (x,y[,z] per particle), the
+ *    layout of Eigen's Vecd in a Real=double build; scalars as double; index
+ *    fields as uint32.  The library converts to its own device layout.
+ *  - Calls on objects of one context are enqueued on that context's stream
+ *    in call order; calls that return host data synchronise the stream.
+ *    Different contexts may be driven from different host threads.
+ */
+#ifndef SPHX_H
+#define SPHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int sphx_status;
+enum {
+    SPHX_OK = 0,
+    SPHX_E_ARG = 1,       /* invalid argument or inconsistent objects          */
+    SPHX_E_HIP = 2,       /* HIP runtime error                                  */
+    SPHX_E_CAPACITY = 3,  /* a caller-provided buffer is too small              */
+    SPHX_E_NONFINITE = 4, /* a non-finite value was detected in the state       */
+    SPHX_E_NOMEM = 5,     /* device allocation failed                           */
+    SPHX_E_STATE = 6      /* call out of order (e.g. relation never updated)    */
+};
+
+/* Arithmetic of a body.  FP64: every field and every pair interaction in
+ * double (the reference's Real=double build).  MIXED: pair interactions,
+ * per-particle fields and their accumulators in float; position,
+ * displacement, physical time and global mass/energy reductions in double. */
+enum { SPHX_FP64 = 0, SPHX_MIXED = 1 };
+
+/* Body kinds (reference: FluidBody / SolidBody / ObserverBody,
+ * src/shared/bodies/base_body.h). */
+enum { SPHX_BODY_FLUID = 0, SPHX_BODY_SOLID = 1, SPHX_BODY_OBSERVER = 2 };
+
+/* Particle fields.  Vector fields carry `dim` doubles per particle. */
+enum {
+    SPHX_FIELD_POSITION = 0,          /* "Position"              Vecd  */
+    SPHX_FIELD_VOLUME = 1,            /* "VolumetricMeasure"     Real  */
+    SPHX_FIELD_MASS = 2,              /* "Mass"                  Real  */
+    SPHX_FIELD_DENSITY = 3,           /* "Density"               Real  */
+    SPHX_FIELD_PRESSURE = 4,          /* "Pressure"              Real  */
+    SPHX_FIELD_VELOCITY = 5,          /* "Velocity"              Vecd  */
+    SPHX_FIELD_FORCE = 6,             /* "Force"                 Vecd  */
+    SPHX_FIELD_FORCE_PRIOR = 7,       /* "ForcePrior"            Vecd  */
+    SPHX_FIELD_DISPLACEMENT = 8,      /* "Displacement"          Vecd  */
+    SPHX_FIELD_COMPRESSION = 9,       /* "Compression"           Real  */
+    SPHX_FIELD_COMPRESSION_RATE = 10, /* "CompressionRate"       Real  */
+    SPHX_FIELD_VOLUME_REF = 11,       /* "VolumetricMeasureRef"  Real  */
+    SPHX_FIELD_COMPRESSION_SUM = 12,  /* "CompressionSummation"  Real  */
+    SPHX_FIELD_NORMAL = 13,           /* "NormalDirection"       Vecd (solid) */
+    SPHX_FIELD_WALL_VELOCITY = 14,    /* "Velocity" of a solid   Vecd  */
+    SPHX_FIELD_WALL_ACCELERATION = 15,/* "Acceleration" of solid Vecd  */
+    SPHX_FIELD_ORIGINAL_ID = 16,      /* "OriginalID"            uint32 */
+    SPHX_FIELD_SORTED_ID = 17,        /* "SortedID"              uint32 */
+    SPHX_FIELD_SORT_SEQUENCE = 18,    /* sort keys after the last sort, uint32 */
+    SPHX_FIELD_INTERPOLATED = 19,     /* observer result of sphx_interpolate_contact */
+    SPHX_FIELD_COUNT = 20
+};
+
+/* Riemann solvers (reference: riemann_solver_ck.h:166-172,
+ * acoustic_step_2nd_half.h:180-188). */
+enum {
+    SPHX_RIEMANN_ACOUSTIC = 0,    /* AcousticRiemannSolverCK (TruncatedLinear, slope 3) */
+    SPHX_RIEMANN_NONE = 1,        /* NoRiemannSolverCK                                   */
+    SPHX_RIEMANN_DISSIPATIVE = 2  /* DissipativeRiemannSolverCK (NoLimiter)              */
+};
+
+/* Density regularization flow types (reference: density_regularization.h:146-184). */
+enum { SPHX_FLOW_INTERNAL = 0, SPHX_FLOW_FREE_SURFACE = 1 };
+
+typedef struct sphx_context sphx_context;
+typedef struct sphx_body sphx_body;
+typedef struct sphx_cell_linked_list sphx_cell_linked_list;
+typedef struct sphx_relation sphx_relation;
+
+typedef struct sphx_body_desc {
+    int dim;                   /* 2 or 3                                          */
+    int precision;             /* SPHX_FP64 or SPHX_MIXED                         */
+    int kind;                  /* SPHX_BODY_*                                     */
+    uint32_t n_particles;      /* TotalRealParticles                              */
+    double spacing_ref;        /* reference particle spacing dp                  */
+    double h_spacing_ratio;    /* h / dp (SPHAdaptation default 1.3)             */
+    double rho0;               /* reference density (fluid)                       */
+    double c0;                 /* reference sound speed (fluid)                   */
+} sphx_body_desc;
+
+/* ---- identification ---------------------------------------------------- */
+const char *sphx_backend_name(void);     /* "hip-gfx950" for the device library */
+const char *sphx_last_error(void);
+
+/* ---- context ------------------------------------------------------------ */
+sphx_status sphx_context_create(int device, sphx_context **out);
+sphx_status sphx_context_destroy(sphx_context *ctx);
+sphx_status sphx_context_synchronize(sphx_context *ctx);
+
+/* ---- bodies and particle state -----------------------------------------
+ * reference: BaseParticles variable registry and DiscreteVariable mirrors,
+ * src/shared/particles/base_particles.h:109-176,
+ * src/shared/common/sphinxsys_variable.h:282-356;
+ * initial values as BaseParticles::initializeBasicDiscreteVariables
+ * (base_particles.cpp:33-40) and MatterMaterial::initializeMatterVariables
+ * (materials/base_material.cpp:35-41): Compression=1, OriginalID=SortedID=i,
+ * Density=rho0, VolumetricMeasureRef=Vol at first upload of Vol. */
+sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *desc, sphx_body **out);
+sphx_status sphx_body_destroy(sphx_body *body);
+sphx_status sphx_body_write(sphx_body *body, int field, const void *host, size_t n_particles);
+sphx_status sphx_body_read(sphx_body *body, int field, void *host, size_t n_particles);
+/* Smoothing length h = h_spacing_ratio * spacing_ref as the body stores it. */
+sphx_status sphx_body_smoothing_length(const sphx_body *body, double *h);
+
+/* ---- cell linked list ----------------------------------------------------
+ * reference: CellLinkedList<SPHAdaptation> over the system bounds with grid
+ * spacing = kernel cut-off radius and a 2-cell buffer,
+ * src/shared/meshes/cell_linked_list.cpp:190-197, base_mesh.cpp:6-16,
+ * adaptation.cpp:80-85, base_body.cpp:99-105. */
+sphx_status sphx_cell_linked_list_create(sphx_body *body, const double *bounds_lower,
+                                         const double *bounds_upper, double grid_spacing,
+                                         uint32_t buffer_width, sphx_cell_linked_list **out);
+sphx_status sphx_cell_linked_list_destroy(sphx_cell_linked_list *cll);
+/* all_cells[dim], mesh_lower_bound[dim], grid_spacing. */
+sphx_status sphx_cell_linked_list_mesh(const sphx_cell_linked_list *cll, int *all_cells,
+                                       double *lower_bound, double *grid_spacing);
+/* reference: UpdateCellLinkedList<P, RealBody>::exec,
+ * shared_ck/particle_dynamics/configuration_dynamics/update_cell_linked_list.hpp:75-106 */
+sphx_status sphx_update_cell_linked_list(sphx_cell_linked_list *cll);
+/* cell_offset[n_cells+1], particle_index[n_particles] (per-cell index order
+ * is ascending particle index; the reference fills it with relaxed atomics). */
+sphx_status sphx_cell_linked_list_read(sphx_cell_linked_list *cll, uint32_t *cell_offset,
+                                       size_t n_cell_offset, uint32_t *particle_index,
+                                       size_t n_particle_index);
+
+/* ---- relations -------------------------------------------------------------
+ * reference: Inner<> / Contact<> (shared_ck/body_relation/relation_ck.h:89-174)
+ * with Neighbor<SPHAdaptation,SPHAdaptation> (neighbor_method.hpp:66-161). */
+sphx_status sphx_relation_create_inner(sphx_body *body, sphx_cell_linked_list *cll,
+                                       sphx_relation **out);
+sphx_status sphx_relation_create_contact(sphx_body *source, sphx_body *target,
+                                         sphx_cell_linked_list *target_cll,
+                                         sphx_relation **out);
+sphx_status sphx_relation_destroy(sphx_relation *rel);
+/* reference: UpdateRelation<P, Inner<>>::exec (update_body_relation.hpp:117-164)
+ *        and UpdateRelation<P, Contact<>>::exec (update_body_relation.hpp:240-288). */
+sphx_status sphx_update_relation(sphx_relation *rel);
+/* neighbor_size[n_src], particle_offset[n_src+1], neighbor_index[*total].
+ * Pass neighbor_index=NULL to query *total only. */
+sphx_status sphx_relation_read(sphx_relation *rel, uint32_t *neighbor_size,
+                               uint32_t *particle_offset, uint32_t *neighbor_index,
+                               size_t capacity, size_t *total);
+
+/* ---- configuration dynamics -------------------------------------------------
+ * reference: ParticleSortCK<P>::exec (configuration_dynamics/particle_sort_ck.hpp:76-104):
+ * Morton keys of the CLL mesh cells, sort, permutation of the evolving
+ * variables (Position, VolumetricMeasure, OriginalID, Velocity, Mass,
+ * ForcePrior, Compression, CompressionRate, VolumetricMeasureRef), SortedID. */
+sphx_status sphx_particle_sort(sphx_body *body, sphx_cell_linked_list *cll);
+
+/* ---- state dynamics ------------------------------------------------------- */
+/* reference: StateDynamics<P, GravityForceCK<Gravity>> (general_dynamics/force_prior_ck.hpp:37-43) */
+sphx_status sphx_gravity_force(sphx_body *fluid, const double *gravity);
+/* reference: DensityRegularization<SPHBody, WeaklyCompressibleFluid, FlowType>
+ * (fluid_dynamics/density_regularization.hpp:85-118) */
+sphx_status sphx_density_regularization(sphx_body *fluid, int flow_type);
+/* reference: AdvectionStepSetup (fluid_dynamics/fluid_time_step_ck.h:133-160) */
+sphx_status sphx_advection_step_setup(sphx_body *fluid);
+/* reference: UpdateParticlePosition (fluid_dynamics/fluid_time_step_ck.h:162-184) */
+sphx_status sphx_update_particle_position(sphx_body *fluid);
+
+/* ---- interaction dynamics ------------------------------------------------- */
+/* reference: InteractionDynamicsCK<P, CompressionSummation<Inner<>, Contact<>>>
+ * (fluid_dynamics/density_regularization.hpp:14-83). contact may be NULL when n_contact=0. */
+sphx_status sphx_compression_summation(sphx_relation *inner, sphx_relation *const *contact,
+                                       int n_contact);
+/* reference: InteractionDynamicsCK<P, AcousticStep1stHalf<Inner<OneLevel,R,NoKernelCorrectionCK>,
+ *                                  Contact<Wall,R,NoKernelCorrectionCK>>>
+ * (fluid_dynamics/acoustic_step_1st_half.hpp:66-180). walls may be NULL when n_wall=0. */
+sphx_status sphx_acoustic_step_1st_half(sphx_relation *inner, sphx_relation *const *walls,
+                                        int n_wall, int riemann, double dt);
+/* reference: ... AcousticStep2ndHalf<...> (fluid_dynamics/acoustic_step_2nd_half.hpp:33-137). */
+sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *const *walls,
+                                        int n_wall, int riemann, double dt);
+
+/* ---- reduce dynamics -------------------------------------------------------- */
+/* reference: ReduceDynamicsCK<P, AcousticTimeStepCK<WeaklyCompressibleFluid>>
+ * (fluid_dynamics/fluid_time_step_ck.hpp:30-57); acoustic_cfl default 0.6. */
+sphx_status sphx_acoustic_time_step(sphx_body *fluid, double acoustic_cfl, double *dt);
+/* reference: ReduceDynamicsCK<P, AdvectionTimeStepCK> (fluid_time_step_ck.cpp:12-26,
+ * fluid_time_step_ck.h:84-120); advection_cfl default 0.25. */
+sphx_status sphx_advection_time_step(sphx_body *fluid, double u_ref, double advection_cfl,
+                                     double *dt);
+/* reference: ReducedQuantityRecording<P, TotalMechanicalEnergyCK>
+ * (general_dynamics/general_reduce_ck.h:69-96, external_force.h:53-56). */
+sphx_status sphx_total_mechanical_energy(sphx_body *fluid, const double *gravity, double *energy);
+/* reference: UpperFrontInAxisDirectionCK (general_dynamics/general_reduce_ck.h:197). */
+sphx_status sphx_upper_front_in_axis(sphx_body *fluid, int axis, double *front);
+
+/* ---- observation -------------------------------------------------------------- */
+/* reference: Interpolation<Contact<Real>> over an observer contact relation
+ * (general_dynamics/interpolation_dynamics.hpp:46-60); field is a scalar field
+ * of the contact body (e.g. SPHX_FIELD_PRESSURE).  Result per observer point
+ * in out[n_observer]. */
+sphx_status sphx_interpolate_contact(sphx_relation *observer_contact, int field, double *out,
+                                     size_t n_observer);
+
+/* ---- dual-time stepping driver ---------------------------------------------------
+ * One advection step of the weakly-compressible loop of the CK dam-break
+ * cases (tests/tests_sycl/3d_examples/test_3d_dambreak_sycl/dambreak.cpp:188-222):
+ * compression summation -> regularization -> setup -> advection dt ->
+ * acoustic sub-steps (dt, 1st half, 2nd half) until their sum reaches the
+ * advection dt -> position update -> [sort when sort_now] -> cell linked
+ * list -> relations.  The acoustic time step size stays on the device.
+ * On return *physical_time has advanced and *acoustic_steps holds the number
+ * of sub-steps taken. No reference counterpart: it is the case-file loop body
+ * executed behind one call. */
+typedef struct sphx_advection_config {
+    double u_ref;          /* AdvectionTimeStepCK U_ref                 */
+    double advection_cfl;  /* 0.25                                        */
+    double acoustic_cfl;   /* 0.6                                         */
+    int riemann;           /* SPHX_RIEMANN_*                              */
+    int flow_type;         /* SPHX_FLOW_*                                 */
+    int sort_now;          /* run ParticleSortCK after the position update */
+} sphx_advection_config;
+
+sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                                sphx_cell_linked_list *fluid_cll,
+                                const sphx_advection_config *cfg, double *physical_time,
+                                uint32_t *acoustic_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHX_H */

@@ -1,0 +1,190 @@
+"""Dam-break cases: geometry, particle generation and the CK time loop.
+
+2D: tests/2d_examples/test_2d_dambreak/Dambreak.cpp and
+    tests/tests_sycl/2d_examples/test_2d_dambreak_sycl/dambreak_sycl.cpp
+    (tank 5.366 x 5.366, column 2 x 1, dp 0.025, BW 4 dp, rho0 1, g 1).
+3D: the CK case tests/tests_sycl/3d_examples/test_3d_dambreak_sycl/dambreak.cpp
+    with the tank/column/material parameters of the 8M-particle workload
+    (tank 3.2 x 1.0 x 1.5 with z up, column 1.2 x 1.0 x 0.6 at x = 0,
+    3 wall layers, rho0 1000, g 9.81, c0 = 10 * 2 sqrt(g 0.6)).
+The loop body is the CK 3D dam-break order (dambreak.cpp:188-222).
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as capi
+from . import ck
+from .geometry import Box, ComplexShape, lattice_positions, normal_from_body_shape, system_bounds
+
+
+@dataclass
+class DamBreakGeometry:
+    dim: int
+    dp: float
+    fluid_pos: np.ndarray
+    wall_pos: np.ndarray
+    wall_normal: np.ndarray
+    bounds_lower: np.ndarray   # system bounds (domain expanded by 4 dp)
+    bounds_upper: np.ndarray
+    observer_pos: np.ndarray
+    rho0: float
+    c0: float
+    gravity: np.ndarray
+    u_ref: float
+    meta: dict = field(default_factory=dict)
+
+
+def dambreak_2d(dp=0.025, DL=5.366, DH=5.366, LL=2.0, LH=1.0, wall_layers=4, rho0=1.0, g=1.0):
+    """Geometry of test_2d_dambreak (Dambreak.cpp:12-47, dambreak_sycl.cpp:12-62)."""
+    BW = dp * wall_layers
+    U_ref = 2.0 * math.sqrt(g * LH)
+    c0 = 10.0 * U_ref
+    water = Box((0.5 * LL, 0.5 * LH), (0.5 * LL, 0.5 * LH))
+    outer_half = (0.5 * DL + BW, 0.5 * DH + BW)
+    wall = ComplexShape().add(Box((-BW + outer_half[0], -BW + outer_half[1]), outer_half)) \
+                         .subtract(Box((0.5 * DL, 0.5 * DH), (0.5 * DL, 0.5 * DH)))
+    lo, up = system_bounds((-BW, -BW), (DL + BW, DH + BW), dp)
+    fluid = lattice_positions(lo, up, dp, water)
+    wall_pos = lattice_positions(lo, up, dp, wall)
+    normals = normal_from_body_shape(wall, wall_pos)
+    return DamBreakGeometry(2, dp, fluid, wall_pos, normals, lo, up,
+                            np.array([[DL, 0.2]]), rho0, c0, np.array([0.0, -g]), U_ref,
+                            {"DL": DL, "DH": DH, "LL": LL, "LH": LH, "BW": BW})
+
+
+def dambreak_3d(dp=0.00448, DL=3.2, DW=1.0, DH=1.5, LL=1.2, LW=1.0, LH=0.6, wall_layers=3,
+                rho0=1000.0, g=9.81, probe=(3.2, 0.5, 0.1)):
+    """3D dam break: x length, y width, z height (gravity along -z)."""
+    BW = dp * wall_layers
+    U_ref = 2.0 * math.sqrt(g * LH)
+    c0 = 10.0 * U_ref
+    half_w = (0.5 * LL, 0.5 * LW, 0.5 * LH)
+    water = Box(half_w, half_w)
+    half_in = (0.5 * DL, 0.5 * DW, 0.5 * DH)
+    half_out = (0.5 * DL + BW, 0.5 * DW + BW, 0.5 * DH + BW)
+    wall = ComplexShape().add(Box(half_in, half_out)).subtract(Box(half_in, half_in))
+    lo, up = system_bounds((-BW, -BW, -BW), (DL + BW, DW + BW, DH + BW), dp)
+    fluid = lattice_positions(lo, up, dp, water)
+    wall_pos = lattice_positions(lo, up, dp, wall)
+    normals = normal_from_body_shape(wall, wall_pos)
+    return DamBreakGeometry(3, dp, fluid, wall_pos, normals, lo, up,
+                            np.array([probe], dtype=np.float64), rho0, c0,
+                            np.array([0.0, 0.0, -g]), U_ref,
+                            {"DL": DL, "DW": DW, "DH": DH, "LL": LL, "LW": LW, "LH": LH, "BW": BW})
+
+
+def dp_for_fluid_count(n_target, LL=1.2, LW=1.0, LH=0.6):
+    """Particle spacing giving about `n_target` fluid particles in the 3D column."""
+    return (LL * LW * LH / float(n_target)) ** (1.0 / 3.0)
+
+
+class DamBreak:
+    """Bodies, relations and CK dynamics of one dam-break scene on one library."""
+
+    def __init__(self, lib, geom, precision=capi.FP64, riemann=capi.RIEMANN_ACOUSTIC, device=0,
+                 ctx=None):
+        self.lib, self.geom, self.precision = lib, geom, precision
+        d = geom.dim
+        self.own_ctx = ctx is None
+        self.ctx = ctx if ctx is not None else ck.Context(lib, device)
+        self.fluid = ck.Body(self.ctx, d, len(geom.fluid_pos), geom.dp, capi.BODY_FLUID, precision,
+                             rho0=geom.rho0, c0=geom.c0)
+        self.wall = ck.Body(self.ctx, d, len(geom.wall_pos), geom.dp, capi.BODY_SOLID, precision,
+                            rho0=geom.rho0, c0=geom.c0)
+        self.observer = ck.Body(self.ctx, d, len(geom.observer_pos), geom.dp, capi.BODY_OBSERVER,
+                                precision, rho0=geom.rho0, c0=geom.c0)
+        vol = geom.dp ** d
+        self.fluid.write("Position", geom.fluid_pos)
+        self.fluid.write("VolumetricMeasure", np.full(len(geom.fluid_pos), vol))
+        self.wall.write("Position", geom.wall_pos)
+        self.wall.write("VolumetricMeasure", np.full(len(geom.wall_pos), vol))
+        self.wall.write("NormalDirection", geom.wall_normal)
+        self.observer.write("Position", geom.observer_pos)
+        self.observer.write("VolumetricMeasure", np.full(len(geom.observer_pos), vol))
+        lo, up = geom.bounds_lower, geom.bounds_upper
+        self.fluid_cll = ck.CellLinkedList(self.fluid, lo, up)
+        self.wall_cll = ck.CellLinkedList(self.wall, lo, up)
+        self.inner = ck.Inner(self.fluid, self.fluid_cll)
+        self.contact = ck.Contact(self.fluid, self.wall, self.wall_cll)
+        self.observer_contact = ck.Contact(self.observer, self.fluid, self.fluid_cll)
+        # dynamics
+        self.fluid_cell_linked_list = ck.UpdateCellLinkedList(self.fluid_cll)
+        self.wall_cell_linked_list = ck.UpdateCellLinkedList(self.wall_cll)
+        self.complex_relation = ck.UpdateRelation(self.inner, self.contact)
+        self.observer_relation = ck.UpdateRelation(self.observer_contact)
+        self.particle_sort = ck.ParticleSortCK(self.fluid, self.fluid_cll)
+        self.gravity = ck.GravityForceCK(self.fluid, geom.gravity)
+        self.summation = ck.CompressionSummation(self.inner, self.contact)
+        self.regularization = ck.DensityRegularization(self.fluid, capi.FLOW_FREE_SURFACE)
+        self.setup = ck.AdvectionStepSetup(self.fluid)
+        self.position = ck.UpdateParticlePosition(self.fluid)
+        self.first_half = ck.AcousticStep1stHalf(self.inner, self.contact, riemann=riemann)
+        self.second_half = ck.AcousticStep2ndHalf(self.inner, self.contact, riemann=riemann)
+        self.acoustic_dt = ck.AcousticTimeStepCK(self.fluid)
+        self.advection_dt = ck.AdvectionTimeStepCK(self.fluid, geom.u_ref)
+        self.energy = ck.TotalMechanicalEnergyCK(self.fluid, geom.gravity)
+        self.probe = ck.InterpolationContact(self.observer_contact, "Pressure")
+        self.front = ck.UpperFrontInAxisDirectionCK(self.fluid, 0)
+        self.advection = ck.AdvectionStep(self.inner, [self.contact], self.fluid_cll, geom.u_ref,
+                                          riemann=riemann)
+        self.time = 0.0
+        self.iterations = 0
+        self.acoustic_steps = 0
+
+    def initialize(self):
+        """Preparation of dambreak.cpp:148-154."""
+        self.gravity.exec()
+        self.fluid_cell_linked_list.exec()
+        self.wall_cell_linked_list.exec()
+        self.complex_relation.exec()
+        self.observer_relation.exec()
+
+    def sort_due(self):
+        # ParticleSortCK every 100 advection steps (dambreak.cpp:213-216)
+        return self.iterations % 100 == 0 and self.iterations != 1
+
+    def advection_step(self, sort=True):
+        """One advection step through the single-call driver; the sort is placed
+        as in the CK 3D case (after the position update when the incremented
+        iteration count is a multiple of 100, dambreak.cpp:210-216)."""
+        due = sort and (self.iterations + 1) % 100 == 0
+        self.time, n = self.advection.exec(self.time, sort_now=due)
+        self.iterations += 1
+        self.acoustic_steps += n
+        return n
+
+    def advection_step_by_dynamics(self, sort=True):
+        """The same step written out with one CK dynamics object per stage."""
+        self.summation.exec()
+        self.regularization.exec()
+        self.setup.exec()
+        adv_dt = self.advection_dt.exec()
+        relaxation, n = 0.0, 0
+        while relaxation < adv_dt:
+            dt = self.acoustic_dt.exec()
+            self.first_half.exec(dt)
+            self.second_half.exec(dt)
+            relaxation += dt
+            self.time += dt
+            n += 1
+        self.position.exec()
+        self.iterations += 1
+        if sort and self.iterations % 100 == 0 and self.iterations != 1:
+            self.particle_sort.exec()
+        self.fluid_cell_linked_list.exec()
+        self.complex_relation.exec()
+        self.acoustic_steps += n
+        return n
+
+    def observe(self):
+        self.observer_relation.exec()
+        return self.probe.exec(), self.energy.exec()
+
+    def close(self):
+        for o in (self.observer_contact, self.contact, self.inner, self.wall_cll, self.fluid_cll,
+                  self.observer, self.wall, self.fluid):
+            o.close()
+        if self.own_ctx:
+            self.ctx.close()

@@ -1,0 +1,1190 @@
+// fluid_kernels.hip -- the weakly-compressible fluid step on gfx950.
+//
+// One thread per fluid particle; each thread walks its own CSR neighbour list
+// (inner, then each wall contact), gathers the neighbour's (x, y, z, Vol)
+// record and pressure or velocity, evaluates the tabulated Wendland C2
+// gradient on the fly (cubic per table interval, table in LDS) and fuses the
+// reference's initialize / interact / update stages of one CK dynamics where
+// the data dependencies allow it.  The gather stencil has no dense
+// contraction, so MFMA is not used; the bound is HBM/L2 traffic and VALU.
+//
+// Reference: shared_ck/particle_dynamics/fluid_dynamics/acoustic_step_1st_half.hpp:66-180,
+// acoustic_step_2nd_half.hpp:33-137, riemann_solver/riemann_solver_ck.hpp:17-69,
+// density_regularization.hpp:40-118, fluid_time_step_ck.{h,hpp,cpp}.
+#include "sphx_internal.hpp"
+
+#include <cmath>
+
+namespace sphx
+{
+namespace
+{
+constexpr int kThreads = 256;
+constexpr int kMaxWalls = 4;
+inline unsigned grid_for(size_t n, size_t per) { return (unsigned)((n + per - 1) / per); }
+
+template <class T>
+__device__ __forceinline__ T inv_sqrt(T r2);
+template <>
+__device__ __forceinline__ float inv_sqrt<float>(float r2)
+{
+    return r2 > 0.0f ? rsqrtf(r2) : 0.0f;
+}
+template <>
+__device__ __forceinline__ double inv_sqrt<double>(double r2)
+{
+    return r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+}
+
+template <class T>
+struct WallArgs
+{
+    const Rec4<T> *posvol; // wall (x, y, z|Vol, Vol|0)
+    const T *vel[3];       // wall "Velocity" (average velocity of the solid)
+    const T *acc[3];       // wall "Acceleration"
+    const T *normal[3];    // wall "NormalDirection"
+    const T *Vol_ref;      // wall "VolumetricMeasureRef"
+    const uint32_t *offset, *index;
+    const KernelPoly<T> *poly;
+};
+
+template <class T>
+struct FluidArgs
+{
+    uint32_t n;
+    const Rec4<T> *posvol;
+    Rec4<T> *vel;
+    T *force[3];
+    const T *force_prior[3];
+    const T *mass;
+    T *rho, *p, *C, *dC, *S;
+    const T *Vol_ref;
+    double *dpos[3];
+    T rho0, c0, p0, h;
+    // single-fluid impedance model (riemann_solver_ck.hpp:59-69)
+    T rho0c0, inv_rho0c0_sum, inv_rho0c0_ave, rho0c0_geo_ave, inv_c0_ave;
+    int riemann;
+    const uint32_t *offset, *index;
+    const KernelPoly<T> *poly;
+    int n_wall;
+    WallArgs<T> wall[kMaxWalls];
+};
+
+// position/volume record of particle j
+template <class T, int D>
+struct PV
+{
+    T x, y, z, vol;
+};
+template <class T, int D>
+__device__ __forceinline__ PV<T, D> load_pv(const Rec4<T> *a, uint32_t j)
+{
+    Rec4<T> r = a[j];
+    if (D == 3)
+        return PV<T, D>{r.x, r.y, r.z, r.w};
+    return PV<T, D>{r.x, r.y, T(0), r.z};
+}
+
+// compression helpers: FP64 stores C, MIXED stores C - 1
+template <int P>
+__device__ __forceinline__ typename Prec<P>::T full_C(typename Prec<P>::T stored)
+{
+    using T = typename Prec<P>::T;
+    return Prec<P>::c_minus_one ? stored + T(1) : stored;
+}
+
+// tabulated kernel polynomial in LDS
+template <class T>
+struct PolyLDS
+{
+    T c[kPolyIntervals][4];
+};
+
+template <class T>
+__device__ __forceinline__ void stage_poly(PolyLDS<T> &lds, const KernelPoly<T> *g, bool grad)
+{
+    for (int k = threadIdx.x; k < kPolyIntervals * 4; k += blockDim.x)
+        lds.c[k >> 2][k & 3] = grad ? g->dw[k >> 2][k & 3] : g->w[k >> 2][k & 3];
+}
+
+template <class T>
+__device__ __forceinline__ T eval_poly(const PolyLDS<T> &lds, T q, T inv_dq, T dq)
+{
+    int loc = (int)(q * inv_dq);
+    loc = loc < 0 ? 0 : (loc > kPolyIntervals - 1 ? kPolyIntervals - 1 : loc);
+    T t = q - T(loc) * dq;
+    const T *c = lds.c[loc];
+    return ((c[3] * t + c[2]) * t + c[1]) * t + c[0];
+}
+
+//---------------------------------------------------------------------------
+// AcousticStep1stHalf::InitializeKernel (acoustic_step_1st_half.hpp:66-74)
+//---------------------------------------------------------------------------
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_init_1st(FluidArgs<typename Prec<P>::T> a, double dt_d)
+{
+    using T = typename Prec<P>::T;
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T dt = T(dt_d);
+    T c = a.C[i] + T(0.5) * dt * a.dC[i];
+    a.C[i] = c;
+    T rho = full_C<P>(c) * a.rho0;
+    a.rho[i] = rho;
+    a.p[i] = Prec<P>::c_minus_one ? a.p0 * c : a.p0 * (rho / a.rho0 - T(1));
+    Rec4<T> v = a.vel[i];
+    a.dpos[0][i] += (double)(v.x * dt * T(0.5));
+    a.dpos[1][i] += (double)(v.y * dt * T(0.5));
+    if (D == 3)
+        a.dpos[2][i] += (double)(v.z * dt * T(0.5));
+}
+
+//---------------------------------------------------------------------------
+// AcousticStep2ndHalf::InitializeKernel (acoustic_step_2nd_half.hpp:33-38)
+//---------------------------------------------------------------------------
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_init_2nd(FluidArgs<typename Prec<P>::T> a, double dt_d)
+{
+    using T = typename Prec<P>::T;
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T dt = T(dt_d);
+    Rec4<T> v = a.vel[i];
+    a.dpos[0][i] += (double)(v.x * dt * T(0.5));
+    a.dpos[1][i] += (double)(v.y * dt * T(0.5));
+    if (D == 3)
+        a.dpos[2][i] += (double)(v.z * dt * T(0.5));
+}
+
+//---------------------------------------------------------------------------
+// pass 1: inner + wall interact of AcousticStep1stHalf, then its update
+// (v += (F_prior + F) / m dt); optionally the 2nd half's initialize.
+//---------------------------------------------------------------------------
+template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND>
+__global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    stage_poly(poly_in, a.poly, true);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, true);
+    __syncthreads();
+    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T dt = T(dt_d);
+    const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    const T p_i = a.p[i];
+    // inner: force_sum -= 2 AverageP dW V_j e_ij, AverageP = (p_i + p_j)/2 for one fluid
+    T fx = 0, fy = 0, fz = 0, cd = 0;
+    const uint32_t n0 = a.offset[i], n1 = a.offset[i + 1];
+    for (uint32_t n = n0; n < n1; ++n)
+    {
+        const uint32_t j = a.index[n];
+        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+        const T p_j = a.p[j];
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T ir = inv_sqrt(r2);
+        const T q = r2 * ir * inv_h;
+        const T dWV = eval_poly(poly_in, q, inv_dq, dq) * pj.vol;
+        const T s = (p_i + p_j) * dWV * ir;
+        fx += s * dx;
+        fy += s * dy;
+        if (D == 3)
+            fz += s * dz;
+        if (RIEMANN)
+            cd += (p_i - p_j) * dWV;
+    }
+    const T k_avg = T(2) * a.inv_rho0c0_sum * a.rho0c0; // 2 * AverageP / (p_i + p_j)
+    const T vol_i = pi.vol;
+    T Fx = a.force[0][i], Fy = a.force[1][i], Fz = D == 3 ? a.force[2][i] : T(0);
+    Fx += -k_avg * fx * vol_i;
+    Fy += -k_avg * fy * vol_i;
+    if (D == 3)
+        Fz += -k_avg * fz * vol_i;
+    const T C_i = full_C<P>(a.C[i]);
+    T dC = RIEMANN ? cd * a.inv_rho0c0_ave * C_i : T(0);
+
+    // walls: p_w = p_i + rho_i r max(0, (Fp_i/m_i - a_w).(-e)); force -= (p_i+p_w) dW V_w e
+    if (a.n_wall > 0)
+    {
+        const T m_i = a.mass[i], rho_i = a.rho[i];
+        const T apx = a.force_prior[0][i] / m_i, apy = a.force_prior[1][i] / m_i;
+        const T apz = D == 3 ? a.force_prior[2][i] / m_i : T(0);
+        for (int k = 0; k < a.n_wall; ++k)
+        {
+            const WallArgs<T> &w = a.wall[k];
+            const T w_inv_h = w.poly->inv_h;
+            T wx = 0, wy = 0, wz = 0, wcd = 0;
+            const uint32_t m0 = w.offset[i], m1 = w.offset[i + 1];
+            for (uint32_t n = m0; n < m1; ++n)
+            {
+                const uint32_t j = w.index[n];
+                const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
+                const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+                const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+                const T ir = inv_sqrt(r2);
+                const T r = r2 * ir;
+                const T dWV = eval_poly(poly_w[k], r * w_inv_h, inv_dq, dq) * pj.vol;
+                const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+                T face = -((apx - w.acc[0][j]) * ex + (apy - w.acc[1][j]) * ey);
+                if (D == 3)
+                    face -= (apz - w.acc[2][j]) * ez;
+                const T p_w = p_i + rho_i * r * dmax(T(0), face);
+                const T s = (p_i + p_w) * dWV;
+                wx += s * ex;
+                wy += s * ey;
+                if (D == 3)
+                    wz += s * ez;
+                if (RIEMANN)
+                    wcd += (p_i - p_w) * dWV;
+            }
+            Fx += -wx * vol_i;
+            Fy += -wy * vol_i;
+            if (D == 3)
+                Fz += -wz * vol_i;
+            if (RIEMANN)
+                dC += wcd * a.inv_rho0c0_ave * C_i;
+        }
+    }
+    a.force[0][i] = Fx;
+    a.force[1][i] = Fy;
+    if (D == 3)
+        a.force[2][i] = Fz;
+    a.dC[i] = dC;
+    // UpdateKernel: v += (F_prior + F) / m * dt
+    const T m_i = a.mass[i];
+    Rec4<T> v = a.vel[i];
+    v.x += (a.force_prior[0][i] + Fx) / m_i * dt;
+    v.y += (a.force_prior[1][i] + Fy) / m_i * dt;
+    if (D == 3)
+        v.z += (a.force_prior[2][i] + Fz) / m_i * dt;
+    a.vel[i] = v;
+    if (FUSE_INIT_2ND)
+    {
+        a.dpos[0][i] += (double)(v.x * dt * T(0.5));
+        a.dpos[1][i] += (double)(v.y * dt * T(0.5));
+        if (D == 3)
+            a.dpos[2][i] += (double)(v.z * dt * T(0.5));
+    }
+}
+
+// limiter of AcousticRiemannSolverCK: TruncatedLinear(3) on inv_c0 * max(u, 0)
+template <class T>
+__device__ __forceinline__ T p_jump(const FluidArgs<T> &a, T u)
+{
+    T lim = a.riemann == SPHX_RIEMANN_ACOUSTIC ? dmin(T(3) * (a.inv_c0_ave * dmax(u, T(0))), T(1)) : T(1);
+    return a.rho0c0_geo_ave * u * lim;
+}
+
+//---------------------------------------------------------------------------
+// pass 2: inner + wall interact of AcousticStep2ndHalf, its update
+// (C += dt/2 dC, rho = C rho0) and, fused, the per-particle signal speed of
+// AcousticTimeStepCK for the next sub-step, max-reduced into *max_slot.
+//---------------------------------------------------------------------------
+template <int D, int P, bool RIEMANN, bool FUSE_CFL>
+__global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d,
+                                                    unsigned long long *max_slot)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    __shared__ double wave_red[kThreads / 64];
+    stage_poly(poly_in, a.poly, true);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, true);
+    __syncthreads();
+    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    double signal = -1.0;
+    if (i < a.n)
+    {
+        const T dt = T(dt_d);
+        const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
+        const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+        const Rec4<T> vi = a.vel[i];
+        // divergence: 2 (v_i - v_ave).e dW V = (v_i - v_j).e dW V for one fluid
+        T div = 0, px = 0, py = 0, pz = 0;
+        const uint32_t n0 = a.offset[i], n1 = a.offset[i + 1];
+        for (uint32_t n = n0; n < n1; ++n)
+        {
+            const uint32_t j = a.index[n];
+            const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+            const Rec4<T> vj = a.vel[j];
+            const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+            const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+            const T ir = inv_sqrt(r2);
+            const T q = r2 * ir * inv_h;
+            const T dWV = eval_poly(poly_in, q, inv_dq, dq) * pj.vol;
+            const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+            T u = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey;
+            if (D == 3)
+                u += (vi.z - vj.z) * ez;
+            div += u * dWV;
+            if (RIEMANN)
+            {
+                const T s = p_jump(a, u) * dWV;
+                px += s * ex;
+                py += s * ey;
+                if (D == 3)
+                    pz += s * ez;
+            }
+        }
+        const T C_i = full_C<P>(a.C[i]);
+        const T vol_i = pi.vol;
+        T dC = a.dC[i] + div * C_i;
+        T Fx = px * vol_i, Fy = py * vol_i, Fz = D == 3 ? pz * vol_i : T(0);
+        for (int k = 0; k < a.n_wall; ++k)
+        {
+            const WallArgs<T> &w = a.wall[k];
+            const T w_inv_h = w.poly->inv_h;
+            T wdiv = 0, wx = 0, wy = 0, wz = 0;
+            const uint32_t m0 = w.offset[i], m1 = w.offset[i + 1];
+            for (uint32_t n = m0; n < m1; ++n)
+            {
+                const uint32_t j = w.index[n];
+                const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
+                const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+                const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+                const T ir = inv_sqrt(r2);
+                const T dWV = eval_poly(poly_w[k], r2 * ir * w_inv_h, inv_dq, dq) * pj.vol;
+                const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+                const T vdx = T(2) * (vi.x - w.vel[0][j]), vdy = T(2) * (vi.y - w.vel[1][j]);
+                const T vdz = D == 3 ? T(2) * (vi.z - w.vel[2][j]) : T(0);
+                T ve = vdx * ex + vdy * ey;
+                if (D == 3)
+                    ve += vdz * ez;
+                wdiv += ve * dWV;
+                if (RIEMANN)
+                {
+                    const T nx = w.normal[0][j], ny = w.normal[1][j], nz = D == 3 ? w.normal[2][j] : T(0);
+                    T en = ex * nx + ey * ny;
+                    if (D == 3)
+                        en += ez * nz;
+                    const T sg = en < T(0) ? T(-1) : (en > T(0) ? T(1) : T(0));
+                    const T fnx = sg * nx, fny = sg * ny, fnz = sg * nz;
+                    T u = vdx * fnx + vdy * fny;
+                    if (D == 3)
+                        u += vdz * fnz;
+                    const T s = p_jump(a, u) * dWV;
+                    wx += s * fnx;
+                    wy += s * fny;
+                    if (D == 3)
+                        wz += s * fnz;
+                }
+            }
+            dC += wdiv * C_i;
+            Fx += wx * vol_i;
+            Fy += wy * vol_i;
+            if (D == 3)
+                Fz += wz * vol_i;
+        }
+        a.dC[i] = dC;
+        a.force[0][i] = Fx;
+        a.force[1][i] = Fy;
+        if (D == 3)
+            a.force[2][i] = Fz;
+        // UpdateKernel: C += 0.5 dt dC; rho = C rho0
+        T c = a.C[i] + T(0.5) * dt * dC;
+        a.C[i] = c;
+        a.rho[i] = full_C<P>(c) * a.rho0;
+        if (FUSE_CFL)
+        {
+            // AcousticTimeStepCK::reduce (fluid_time_step_ck.hpp:51-57)
+            const T m_i = a.mass[i];
+            T gx = Fx + a.force_prior[0][i], gy = Fy + a.force_prior[1][i];
+            T gz = D == 3 ? Fz + a.force_prior[2][i] : T(0);
+            T fn = sqrt(D == 3 ? gx * gx + gy * gy + gz * gz : gx * gx + gy * gy);
+            T acc_scale = sqrt(T(4) * a.h * fn / m_i);
+            T vn = sqrt(D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y);
+            signal = (double)dmax(a.c0 + vn, acc_scale);
+        }
+    }
+    if (FUSE_CFL)
+    {
+        double m = wave_max(signal);
+        if ((threadIdx.x & 63) == 0)
+            wave_red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            double b = wave_red[0];
+            for (int w = 1; w < kThreads / 64; ++w)
+                b = fmax(b, wave_red[w]);
+            atomicMax(max_slot, ordered_bits(b));
+        }
+    }
+}
+
+//---------------------------------------------------------------------------
+// CompressionSummation<Inner<>> + <Contact<>> (density_regularization.hpp:40-83)
+//---------------------------------------------------------------------------
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<P>::T> a)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    stage_poly(poly_in, a.poly, false);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, false);
+    __syncthreads();
+    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    T s = a.poly->w0 * a.Vol_ref[i];
+    const uint32_t n0 = a.offset[i], n1 = a.offset[i + 1];
+    for (uint32_t n = n0; n < n1; ++n)
+    {
+        const uint32_t j = a.index[n];
+        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T q = sqrt(r2) * inv_h;
+        s += eval_poly(poly_in, q, inv_dq, dq) * a.Vol_ref[j];
+    }
+    for (int k = 0; k < a.n_wall; ++k)
+    {
+        const WallArgs<T> &w = a.wall[k];
+        const T w_inv_h = w.poly->inv_h;
+        for (uint32_t n = w.offset[i]; n < w.offset[i + 1]; ++n)
+        {
+            const uint32_t j = w.index[n];
+            const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
+            const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+            const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+            s += eval_poly(poly_w[k], sqrt(r2) * w_inv_h, inv_dq, dq) * w.Vol_ref[j];
+        }
+    }
+    a.S[i] = s;
+}
+
+//---------------------------------------------------------------------------
+// state dynamics
+//---------------------------------------------------------------------------
+template <int D, int P>
+__global__ void k_regularize(FluidArgs<typename Prec<P>::T> a, int flow)
+{
+    using T = typename Prec<P>::T;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n)
+        return;
+    // Regularization<FreeSurface>: C = max(S, 1); <Internal>: C = S
+    T S = a.S[i];
+    T c = flow == SPHX_FLOW_FREE_SURFACE ? dmax(S, T(1)) : S;
+    a.C[i] = Prec<P>::c_minus_one ? c - T(1) : c;
+    a.rho[i] = c * a.rho0;
+}
+
+struct PosArgs
+{
+    double *pos[3], *dpos[3];
+    uint32_t n;
+};
+
+// AdvectionStepSetup (fluid_time_step_ck.h:148-152), fused with the refresh
+// of the (x, y, z, Vol) gather record
+template <int D, int P>
+__global__ void k_setup(FluidArgs<typename Prec<P>::T> a, PosArgs pa, typename Prec<P>::T *Vol,
+                        Rec4<typename Prec<P>::T> *posvol)
+{
+    using T = typename Prec<P>::T;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n)
+        return;
+    T vol = a.mass[i] / a.rho[i];
+    Vol[i] = vol;
+    Rec4<T> r;
+    r.x = T(pa.pos[0][i]);
+    r.y = T(pa.pos[1][i]);
+    if (D == 3)
+    {
+        r.z = T(pa.pos[2][i]);
+        r.w = vol;
+        pa.dpos[2][i] = 0.0;
+    }
+    else
+    {
+        r.z = vol;
+        r.w = T(0);
+    }
+    posvol[i] = r;
+    pa.dpos[0][i] = 0.0;
+    pa.dpos[1][i] = 0.0;
+}
+
+// UpdateParticlePosition (fluid_time_step_ck.h:176-179) + record refresh
+template <int D, class T>
+__global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pa.n)
+        return;
+    Rec4<T> r;
+    double x = pa.pos[0][i] + pa.dpos[0][i];
+    double y = pa.pos[1][i] + pa.dpos[1][i];
+    pa.pos[0][i] = x;
+    pa.pos[1][i] = y;
+    r.x = T(x);
+    r.y = T(y);
+    if (D == 3)
+    {
+        double z = pa.pos[2][i] + pa.dpos[2][i];
+        pa.pos[2][i] = z;
+        r.z = T(z);
+        r.w = Vol[i];
+    }
+    else
+    {
+        r.z = Vol[i];
+        r.w = T(0);
+    }
+    posvol[i] = r;
+}
+
+template <int D, class T>
+__global__ void k_refresh_posvol(PosArgs pa, const T *Vol, Rec4<T> *posvol)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pa.n)
+        return;
+    Rec4<T> r;
+    r.x = T(pa.pos[0][i]);
+    r.y = T(pa.pos[1][i]);
+    if (D == 3)
+    {
+        r.z = T(pa.pos[2][i]);
+        r.w = Vol[i];
+    }
+    else
+    {
+        r.z = Vol[i];
+        r.w = T(0);
+    }
+    posvol[i] = r;
+}
+
+template <int D, class T>
+__global__ void k_gravity(uint32_t n, T *fp0, T *fp1, T *fp2, const T *mass, double g0, double g1, double g2)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    // GravityForceCK + ForcePriorCK::update: force_prior += m g - previous(=0)
+    fp0[i] += mass[i] * T(g0);
+    fp1[i] += mass[i] * T(g1);
+    if (D == 3)
+        fp2[i] += mass[i] * T(g2);
+}
+
+//---------------------------------------------------------------------------
+// reductions
+//---------------------------------------------------------------------------
+__device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
+{
+    __shared__ double red[kThreads / 64];
+    double m = wave_max(v);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double b = red[0];
+        for (int w = 1; w < kThreads / 64; ++w)
+            b = fmax(b, red[w]);
+        atomicMax(slot, ordered_bits(b));
+    }
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_signal_max(FluidArgs<typename Prec<P>::T> a,
+                                                         unsigned long long *slot)
+{
+    using T = typename Prec<P>::T;
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v = -1.0;
+    if (i < a.n)
+    {
+        const T m_i = a.mass[i];
+        T gx = a.force[0][i] + a.force_prior[0][i], gy = a.force[1][i] + a.force_prior[1][i];
+        T gz = D == 3 ? a.force[2][i] + a.force_prior[2][i] : T(0);
+        T fn = sqrt(D == 3 ? gx * gx + gy * gy + gz * gz : gx * gx + gy * gy);
+        T acc_scale = sqrt(T(4) * a.h * fn / m_i);
+        Rec4<T> vi = a.vel[i];
+        T vn = sqrt(D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y);
+        v = (double)dmax(a.c0 + vn, acc_scale);
+    }
+    block_max_to(v, slot);
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_speed_max(FluidArgs<typename Prec<P>::T> a,
+                                                        unsigned long long *slot)
+{
+    using T = typename Prec<P>::T;
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v = -1.0;
+    if (i < a.n)
+    {
+        Rec4<T> vi = a.vel[i];
+        v = (double)(D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y);
+    }
+    block_max_to(v, slot);
+}
+
+__global__ void __launch_bounds__(kThreads) k_axis_max(const double *x, uint32_t n, unsigned long long *slot)
+{
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v = i < n ? x[i] : -1.79769313486231570e308;
+    block_max_to(v, slot);
+}
+
+// TotalMechanicalEnergyCK: sum 0.5 m |v|^2 + m g.(0 - x), fixed-order two-stage sum
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_energy_parts(FluidArgs<typename Prec<P>::T> a, PosArgs pa,
+                                                           double g0, double g1, double g2, double *parts)
+{
+    using T = typename Prec<P>::T;
+    __shared__ double red[kThreads / 64];
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double e = 0.0;
+    if (i < a.n)
+    {
+        Rec4<T> vi = a.vel[i];
+        T m = a.mass[i];
+        T v2 = D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y;
+        double pot = g0 * (0.0 - pa.pos[0][i]) + g1 * (0.0 - pa.pos[1][i]);
+        if (D == 3)
+            pot += g2 * (0.0 - pa.pos[2][i]);
+        e = (double)(T(0.5) * m * v2) + (double)m * pot;
+    }
+    double s = wave_sum(e);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double b = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w)
+            b += red[w];
+        parts[blockIdx.x] = b;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_sum_parts(const double *parts, uint32_t nparts, double *out)
+{
+    __shared__ double red[kThreads / 64];
+    double s = 0.0;
+    for (uint32_t k = threadIdx.x; k < nparts; k += kThreads)
+        s += parts[k];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double b = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w)
+            b += red[w];
+        *out = b;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// Interpolation<Contact<Real>>::interact (interpolation_dynamics.hpp:46-60)
+template <int D, class T>
+__global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> *tar, const T *data,
+                              const uint32_t *offset, const uint32_t *index, const KernelPoly<T> *poly, T *out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_obs)
+        return;
+    const PV<T, D> pi = load_pv<T, D>(obs, i);
+    T q = 0, w = 0;
+    for (uint32_t n = offset[i]; n < offset[i + 1]; ++n)
+    {
+        const uint32_t j = index[n];
+        const PV<T, D> pj = load_pv<T, D>(tar, j);
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T qq = sqrt(r2) * poly->inv_h;
+        int loc = (int)(qq * poly->inv_dq);
+        loc = loc < 0 ? 0 : (loc > kPolyIntervals - 1 ? kPolyIntervals - 1 : loc);
+        const T t = qq - T(loc) * poly->dq;
+        const T *c = poly->w[loc];
+        const T W = ((c[3] * t + c[2]) * t + c[1]) * t + c[0];
+        const T weight = W * pj.vol;
+        q += weight * data[j];
+        w += weight;
+    }
+    out[i] = q / (w + T(2.71051e-20));
+}
+
+} // namespace
+
+//=============================================================================
+// host side
+//=============================================================================
+template <class T>
+static FluidArgs<T> fluid_args(sphx_body *b)
+{
+    FluidArgs<T> a{};
+    a.n = b->n;
+    a.posvol = (const Rec4<T> *)b->posvol;
+    a.vel = (Rec4<T> *)b->vel;
+    for (int k = 0; k < 3; ++k)
+    {
+        a.force[k] = (T *)b->force[k];
+        a.force_prior[k] = (const T *)b->force_prior[k];
+        a.dpos[k] = b->dpos[k];
+    }
+    a.mass = (const T *)b->mass;
+    a.rho = (T *)b->rho;
+    a.p = (T *)b->p;
+    a.C = (T *)b->C;
+    a.dC = (T *)b->dC;
+    a.S = (T *)b->S;
+    a.Vol_ref = (const T *)b->Vol_ref;
+    const double rho0 = b->desc.rho0, c0 = b->desc.c0;
+    a.rho0 = T(rho0);
+    a.c0 = T(c0);
+    a.p0 = T(rho0 * c0 * c0);
+    a.h = T(b->h);
+    // ImpedanceModel for fluid_i == fluid_j (riemann_solver_ck.hpp:59-69), in double
+    const double z = rho0 * c0;
+    const double inv_sum = 1.0 / (z + z);
+    const double inv_ave = (z + z) / (z * z + z * z);
+    a.rho0c0 = T(z);
+    a.inv_rho0c0_sum = T(inv_sum);
+    a.inv_rho0c0_ave = T(inv_ave);
+    a.rho0c0_geo_ave = T(2.0 * z * z * inv_sum);
+    a.inv_c0_ave = T(0.5 * (rho0 + rho0) * inv_ave);
+    return a;
+}
+
+template <class T>
+static void attach_relations(FluidArgs<T> &a, sphx_relation *inner, sphx_relation *const *w, int nw)
+{
+    if (inner)
+    {
+        a.offset = inner->particle_offset;
+        a.index = inner->neighbor_index;
+        a.poly = (const KernelPoly<T> *)inner->poly;
+    }
+    a.n_wall = nw;
+    for (int k = 0; k < nw; ++k)
+    {
+        sphx_body *wb = w[k]->target;
+        WallArgs<T> &x = a.wall[k];
+        x.posvol = (const Rec4<T> *)wb->posvol;
+        for (int d = 0; d < 3; ++d)
+        {
+            x.vel[d] = (const T *)wb->wall_vel[d];
+            x.acc[d] = (const T *)wb->wall_acc[d];
+            x.normal[d] = (const T *)wb->normal[d];
+        }
+        x.Vol_ref = (const T *)wb->Vol_ref;
+        x.offset = w[k]->particle_offset;
+        x.index = w[k]->neighbor_index;
+        x.poly = (const KernelPoly<T> *)w[k]->poly;
+    }
+}
+
+static PosArgs pos_args(sphx_body *b)
+{
+    PosArgs p{};
+    for (int k = 0; k < 3; ++k)
+    {
+        p.pos[k] = b->pos[k];
+        p.dpos[k] = b->dpos[k];
+    }
+    p.n = b->n;
+    return p;
+}
+
+// dispatch a templated launcher over (D, P)
+template <template <int, int> class L, class... A>
+static sphx_status dispatch(sphx_body *b, A &&...args)
+{
+    if (b->D == 2 && b->P == P_FP64)
+        return L<2, P_FP64>::run(b, args...);
+    if (b->D == 3 && b->P == P_FP64)
+        return L<3, P_FP64>::run(b, args...);
+    if (b->D == 2 && b->P == P_MIXED)
+        return L<2, P_MIXED>::run(b, args...);
+    return L<3, P_MIXED>::run(b, args...);
+}
+
+#define LAUNCH_CHECK()                     \
+    do                                     \
+    {                                      \
+        SPHX_HIP(hipGetLastError());       \
+    } while (0)
+
+template <int D, int P>
+struct RefreshL
+{
+    static sphx_status run(sphx_body *b)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_refresh_posvol<D, T>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
+                           pos_args(b), (const T *)b->Vol, (Rec4<T> *)b->posvol);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status refresh_posvol(sphx_body *b) { return dispatch<RefreshL>(b); }
+
+template <int D, int P>
+struct GravityL
+{
+    static sphx_status run(sphx_body *b, const double *g)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_gravity<D, T>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream, b->n,
+                           (T *)b->force_prior[0], (T *)b->force_prior[1], (T *)b->force_prior[2],
+                           (const T *)b->mass, g[0], g[1], D == 3 ? g[2] : 0.0);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status gravity_force(sphx_body *b, const double *g) { return dispatch<GravityL>(b, g); }
+
+template <int D, int P>
+struct RegularizeL
+{
+    static sphx_status run(sphx_body *b, int flow)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_regularize<D, P>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
+                           fluid_args<T>(b), flow);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status density_regularization(sphx_body *b, int flow) { return dispatch<RegularizeL>(b, flow); }
+
+template <int D, int P>
+struct SetupL
+{
+    static sphx_status run(sphx_body *b)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_setup<D, P>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
+                           fluid_args<T>(b), pos_args(b), (T *)b->Vol, (Rec4<T> *)b->posvol);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status advection_step_setup(sphx_body *b) { return dispatch<SetupL>(b); }
+
+template <int D, int P>
+struct PositionL
+{
+    static sphx_status run(sphx_body *b)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_update_position<D, T>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
+                           pos_args(b), (const T *)b->Vol, (Rec4<T> *)b->posvol);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status update_position(sphx_body *b) { return dispatch<PositionL>(b); }
+
+template <int D, int P>
+struct SummationL
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *c, int nc)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        attach_relations(a, inner, c, nc);
+        hipLaunchKernelGGL((k_summation<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
+                           b->ctx->stream, a);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status compression_summation(sphx_relation *inner, sphx_relation *const *c, int nc)
+{
+    return dispatch<SummationL>(inner->source, inner, c, nc);
+}
+
+template <int D, int P>
+struct Acoustic1L
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                           double dt, bool fuse_init_2nd)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        a.riemann = riemann;
+        attach_relations(a, inner, w, nw);
+        dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
+        hipLaunchKernelGGL((k_init_1st<D, P>), g, blk, 0, b->ctx->stream, a, dt);
+        const bool rs = riemann != SPHX_RIEMANN_NONE;
+        if (rs && fuse_init_2nd)
+            hipLaunchKernelGGL((k_pass1<D, P, true, true>), g, blk, 0, b->ctx->stream, a, dt);
+        else if (rs)
+            hipLaunchKernelGGL((k_pass1<D, P, true, false>), g, blk, 0, b->ctx->stream, a, dt);
+        else if (fuse_init_2nd)
+            hipLaunchKernelGGL((k_pass1<D, P, false, true>), g, blk, 0, b->ctx->stream, a, dt);
+        else
+            hipLaunchKernelGGL((k_pass1<D, P, false, false>), g, blk, 0, b->ctx->stream, a, dt);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+template <int D, int P>
+struct Acoustic2L
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                           double dt, bool run_init, unsigned long long *max_slot)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        a.riemann = riemann;
+        attach_relations(a, inner, w, nw);
+        dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
+        if (run_init)
+            hipLaunchKernelGGL((k_init_2nd<D, P>), g, blk, 0, b->ctx->stream, a, dt);
+        const bool rs = riemann != SPHX_RIEMANN_NONE;
+        if (rs && max_slot)
+            hipLaunchKernelGGL((k_pass2<D, P, true, true>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        else if (rs)
+            hipLaunchKernelGGL((k_pass2<D, P, true, false>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        else if (max_slot)
+            hipLaunchKernelGGL((k_pass2<D, P, false, true>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        else
+            hipLaunchKernelGGL((k_pass2<D, P, false, false>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+sphx_status acoustic_1st_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann, double dt,
+                              bool fuse_init_2nd)
+{
+    return dispatch<Acoustic1L>(inner->source, inner, w, nw, riemann, dt, fuse_init_2nd);
+}
+
+sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann, double dt,
+                              bool run_init, bool fuse_cfl)
+{
+    sphx_body *b = inner->source;
+    unsigned long long *slot = fuse_cfl ? b->ctx->scalars_d + SLOT_MAX_A : nullptr;
+    return dispatch<Acoustic2L>(b, inner, w, nw, riemann, dt, run_init, slot);
+}
+
+sphx_status fill_u64(sphx_context *ctx, unsigned long long *p, size_t n, unsigned long long v)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, p, n, v);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+template <int D, int P>
+struct SignalL
+{
+    static sphx_status run(sphx_body *b, unsigned long long *slot)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_signal_max<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
+                           b->ctx->stream, fluid_args<T>(b), slot);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status acoustic_signal_max(sphx_body *b, unsigned long long *slot) { return dispatch<SignalL>(b, slot); }
+
+template <int D, int P>
+struct SpeedL
+{
+    static sphx_status run(sphx_body *b, unsigned long long *slot)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        hipLaunchKernelGGL((k_speed_max<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
+                           b->ctx->stream, fluid_args<T>(b), slot);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status advection_speed_max(sphx_body *b, unsigned long long *slot) { return dispatch<SpeedL>(b, slot); }
+
+sphx_status axis_max(sphx_body *b, int axis, unsigned long long *slot)
+{
+    if (!b->n)
+        return SPHX_OK;
+    hipLaunchKernelGGL(k_axis_max, dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0, b->ctx->stream,
+                       b->pos[axis], b->n, slot);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+template <int D, int P>
+struct EnergyL
+{
+    static sphx_status run(sphx_body *b, const double *g, double *out)
+    {
+        using T = typename Prec<P>::T;
+        uint32_t nb = grid_for(b->n, kThreads);
+        sphx_status s = b->ctx->reduce_parts.reserve((size_t)std::max(1u, nb) * sizeof(double));
+        if (s)
+            return s;
+        double *parts = (double *)b->ctx->reduce_parts.ptr;
+        if (b->n)
+            hipLaunchKernelGGL((k_energy_parts<D, P>), dim3(nb), dim3(kThreads), 0, b->ctx->stream, fluid_args<T>(b),
+                               pos_args(b), g[0], g[1], D == 3 ? g[2] : 0.0, parts);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kThreads), 0, b->ctx->stream, parts, b->n ? nb : 0u, out);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status mechanical_energy(sphx_body *b, const double *g, double *out) { return dispatch<EnergyL>(b, g, out); }
+
+template <int D, int P>
+struct InterpL
+{
+    static sphx_status run(sphx_body *o, sphx_relation *rel, int field)
+    {
+        using T = typename Prec<P>::T;
+        if (!o->n)
+            return SPHX_OK;
+        sphx_body *t = rel->target;
+        const T *data = field == SPHX_FIELD_PRESSURE ? (const T *)t->p
+                        : field == SPHX_FIELD_DENSITY ? (const T *)t->rho
+                                                      : (const T *)t->C;
+        hipLaunchKernelGGL((k_interpolate<D, T>), dim3(grid_for(o->n, 64)), dim3(64), 0, o->ctx->stream,
+                           (const Rec4<T> *)o->posvol, o->n, (const Rec4<T> *)t->posvol, data, rel->particle_offset,
+                           rel->neighbor_index, (const KernelPoly<T> *)rel->poly, (T *)o->interp);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+sphx_status interpolate_contact(sphx_relation *rel, int field) { return dispatch<InterpL>(rel->source, rel, field); }
+
+//---------------------------------------------------------------------------
+// kernel table -> per-interval cubic (host, long double)
+// reference table: kernel_tabulated_ck.cpp:6-25, Wendland C2 kernel_wendland_c2.cpp:8-60
+//---------------------------------------------------------------------------
+template <class T>
+static void expand_table(const double *data, long double factor, T (*out)[4], double dq)
+{
+    const long double d = dq;
+    const long double dl0 = (-1.0 * dq) * (-2.0 * dq) * (-3.0 * dq);
+    const long double dl1 = dq * (-1.0 * dq) * (-2.0 * dq);
+    const long double dl2 = (2.0 * dq) * dq * (-1.0 * dq);
+    const long double dl3 = (3.0 * dq) * (2.0 * dq) * dq;
+    for (int loc = 0; loc < kPolyIntervals; ++loc)
+    {
+        const int i = loc + 1;
+        // basis polynomials in t (t = fraction_1): roots as in interpolateCubic
+        // L0 = t (t-d)(t-2d)/dl0, L1 = (t+d)(t-d)(t-2d)/dl1, L2 = (t+d) t (t-2d)/dl2, L3 = (t+d) t (t-d)/dl3
+        auto cubic = [](long double r0, long double r1, long double r2, long double *c) {
+            // (t - r0)(t - r1)(t - r2) = t^3 - (r0+r1+r2) t^2 + (r0 r1 + r0 r2 + r1 r2) t - r0 r1 r2
+            c[3] = 1.0L;
+            c[2] = -(r0 + r1 + r2);
+            c[1] = r0 * r1 + r0 * r2 + r1 * r2;
+            c[0] = -r0 * r1 * r2;
+        };
+        long double b0[4], b1[4], b2[4], b3[4];
+        cubic(0.0L, d, 2 * d, b0);
+        cubic(-d, d, 2 * d, b1);
+        cubic(-d, 0.0L, 2 * d, b2);
+        cubic(-d, 0.0L, d, b3);
+        for (int k = 0; k < 4; ++k)
+        {
+            long double v = b0[k] / dl0 * data[i - 1] + b1[k] / dl1 * data[i] + b2[k] / dl2 * data[i + 1] +
+                            b3[k] / dl3 * data[i + 2];
+            out[loc][k] = T(v * factor);
+        }
+    }
+}
+
+template <class T>
+static sphx_status build_poly_t(sphx_relation *rel)
+{
+    sphx_body *src = rel->source;
+    const int D = src->D;
+    // kernel of the source body (h of the source adaptation), base_kernel.cpp:6-27
+    const double h = src->h;
+    const double inv_h_k = 1.0 / h;
+    const double Pi = M_PI;
+    const double factor_W_2D = inv_h_k * inv_h_k * 7.0 / (4.0 * Pi);
+    const double factor_W_3D = inv_h_k * inv_h_k * inv_h_k * 21.0 / (16.0 * Pi);
+    const double dimfac = D == 2 ? factor_W_2D * h * h : factor_W_3D * h * h * h;
+    const double kernel_size = 2.0;
+    const double dq = kernel_size / 20.0;
+    double w[24], dw[24];
+    for (int i = 0; i < 24; ++i)
+    {
+        double q = double(i - 1) * dq;
+        w[i] = std::pow(1.0 - 0.5 * q, 4) * (1.0 + 2.0 * q);
+        dw[i] = 0.625 * std::pow(q - 2.0, 3) * q;
+    }
+    // Neighbor<SPHAdaptation,SPHAdaptation> factors, neighbor_method.hpp:66-95
+    const double inv_h = rel->inv_h, src_inv_h = rel->src_inv_h;
+    const double inv_h2 = inv_h * inv_h, inv_h3 = inv_h2 * inv_h, inv_h4 = inv_h3 * inv_h;
+    const double src_inv_h2 = src_inv_h * src_inv_h, src_inv_h3 = src_inv_h2 * src_inv_h;
+    const double fW = (D == 2 ? inv_h2 : inv_h3) * dimfac;
+    const double fdW = (D == 2 ? inv_h3 : inv_h4) * dimfac;
+    KernelPoly<T> kp;
+    expand_table(w, (long double)fW, kp.w, dq);
+    expand_table(dw, (long double)fdW, kp.dw, dq);
+    kp.inv_h = T(inv_h);
+    kp.dq = T(dq);
+    kp.inv_dq = T(1.0 / dq);
+    // W0 = src_inv_h^D dimfac W(0); interpolateCubic at q = 0 returns data[1] scaled by the basis
+    const double f1 = 0.0, f0 = dq, f2 = -dq, f3 = -2 * dq;
+    const double dl0 = (-1.0 * dq) * (-2.0 * dq) * (-3.0 * dq), dl1 = dq * (-1.0 * dq) * (-2.0 * dq);
+    const double dl2 = (2.0 * dq) * dq * (-1.0 * dq), dl3 = (3.0 * dq) * (2.0 * dq) * dq;
+    const double interp0 = (f1 * f2 * f3) / dl0 * w[0] + (f0 * f2 * f3) / dl1 * w[1] +
+                           (f0 * f1 * f3) / dl2 * w[2] + (f0 * f1 * f2) / dl3 * w[3];
+    kp.w0 = T((D == 2 ? src_inv_h2 : src_inv_h3) * dimfac * interp0);
+    if (!rel->poly)
+        SPHX_HIP(hipMalloc(&rel->poly, sizeof(KernelPoly<T>)));
+    SPHX_HIP(hipMemcpyAsync(rel->poly, &kp, sizeof(kp), hipMemcpyHostToDevice, src->ctx->stream));
+    SPHX_HIP(hipStreamSynchronize(src->ctx->stream));
+    return SPHX_OK;
+}
+
+sphx_status build_kernel_poly(sphx_relation *rel)
+{
+    return rel->source->P == P_FP64 ? build_poly_t<double>(rel) : build_poly_t<float>(rel);
+}
+
+} // namespace sphx

@@ -1,0 +1,958 @@
+// sphx_api.hip -- the sphx C ABI (include/sphx.h) on HIP for gfx950.
+//
+// Object lifetime, argument validation, host<->device layout conversion and
+// the order of kernel launches that restates each CK dynamics' exec().
+// There is no CPU fallback: every compute entry point launches device code.
+#include "sphx_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+namespace sphx
+{
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+sphx_status hip_fail(hipError_t e, const char *what)
+{
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? SPHX_E_NOMEM : SPHX_E_HIP;
+}
+static sphx_status arg_error(const std::string &msg)
+{
+    g_error = msg;
+    return SPHX_E_ARG;
+}
+static sphx_status state_error(const std::string &msg)
+{
+    g_error = msg;
+    return SPHX_E_STATE;
+}
+
+sphx_status Scratch::reserve(size_t n)
+{
+    if (n <= bytes)
+        return SPHX_OK;
+    if (ptr)
+    {
+        // the old buffer may still be in use by queued kernels
+        SPHX_HIP(hipDeviceSynchronize());
+        SPHX_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    size_t want = n + n / 4;
+    SPHX_HIP(hipMalloc(&ptr, want));
+    bytes = want;
+    return SPHX_OK;
+}
+Scratch::~Scratch()
+{
+    if (ptr)
+        (void)hipFree(ptr);
+}
+
+sphx_status read_scalars(sphx_context *ctx, int first, int count)
+{
+    SPHX_HIP(hipMemcpyAsync(ctx->scalars_h + first, ctx->scalars_d + first, count * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    return SPHX_OK;
+}
+
+template <class T>
+sphx_status transfer_scalar_values(sphx_body *b, void *dev, double *h, bool write)
+{
+    const size_t n = b->n;
+    hipStream_t s = b->ctx->stream;
+    std::vector<T> tmp(n);
+    if (write)
+    {
+        for (size_t i = 0; i < n; ++i)
+            tmp[i] = T(h[i]);
+        SPHX_HIP(hipMemcpyAsync(dev, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+        SPHX_HIP(hipStreamSynchronize(s));
+    }
+    else
+    {
+        SPHX_HIP(hipMemcpyAsync(tmp.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        SPHX_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < n; ++i)
+            h[i] = (double)tmp[i];
+    }
+    return SPHX_OK;
+}
+
+namespace
+{
+template <class T>
+__global__ void k_fill(T *a, size_t n, T v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
+template <class T>
+sphx_status fill(sphx_context *ctx, void *p, size_t n, T v)
+{
+    if (n == 0)
+        return SPHX_OK;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (T *)p, n, v);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+sphx_status alloc(sphx_body *b, void **p, size_t bytes)
+{
+    SPHX_HIP(hipMalloc(p, std::max<size_t>(bytes, 64)));
+    b->allocations.push_back(*p);
+    return SPHX_OK;
+}
+
+constexpr unsigned long long kOrderedLowest = 0x0010000000000000ull; // ordered_bits(-DBL_MAX)
+
+// host <-> device transfer of one field with layout/precision conversion
+template <class T>
+sphx_status transfer(sphx_body *b, int field, void *host, bool write)
+{
+    const size_t n = b->n;
+    const int D = b->D;
+    hipStream_t s = b->ctx->stream;
+    double *h = (double *)host;
+    auto scalar_dev = [&](void *dev) -> sphx_status {
+        std::vector<T> tmp(n);
+        if (write)
+        {
+            for (size_t i = 0; i < n; ++i)
+                tmp[i] = T(h[i]);
+            SPHX_HIP(hipMemcpyAsync(dev, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+            SPHX_HIP(hipStreamSynchronize(s));
+        }
+        else
+        {
+            SPHX_HIP(hipMemcpyAsync(tmp.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, s));
+            SPHX_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i)
+                h[i] = (double)tmp[i];
+        }
+        return SPHX_OK;
+    };
+    auto soa_dev = [&](void **dev, auto zero) -> sphx_status {
+        using E = decltype(zero);
+        std::vector<E> tmp(n);
+        for (int k = 0; k < D; ++k)
+        {
+            if (write)
+            {
+                for (size_t i = 0; i < n; ++i)
+                    tmp[i] = E(h[i * D + k]);
+                SPHX_HIP(hipMemcpyAsync(dev[k], tmp.data(), n * sizeof(E), hipMemcpyHostToDevice, s));
+                SPHX_HIP(hipStreamSynchronize(s));
+            }
+            else
+            {
+                SPHX_HIP(hipMemcpyAsync(tmp.data(), dev[k], n * sizeof(E), hipMemcpyDeviceToHost, s));
+                SPHX_HIP(hipStreamSynchronize(s));
+                for (size_t i = 0; i < n; ++i)
+                    h[i * D + k] = (double)tmp[i];
+            }
+        }
+        return SPHX_OK;
+    };
+    auto u32_dev = [&](uint32_t *dev) -> sphx_status {
+        if (write)
+            SPHX_HIP(hipMemcpyAsync(dev, host, n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        else
+            SPHX_HIP(hipMemcpyAsync(host, dev, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SPHX_HIP(hipStreamSynchronize(s));
+        return SPHX_OK;
+    };
+    const bool cm1 = b->P == P_MIXED; // MIXED stores C - 1
+    switch (field)
+    {
+    case SPHX_FIELD_POSITION:
+        return soa_dev((void **)b->pos, 0.0);
+    case SPHX_FIELD_DISPLACEMENT:
+        return soa_dev((void **)b->dpos, 0.0);
+    case SPHX_FIELD_VOLUME:
+        return scalar_dev(b->Vol);
+    case SPHX_FIELD_MASS:
+        return scalar_dev(b->mass);
+    case SPHX_FIELD_DENSITY:
+        return scalar_dev(b->rho);
+    case SPHX_FIELD_PRESSURE:
+        return scalar_dev(b->p);
+    case SPHX_FIELD_COMPRESSION:
+    {
+        std::vector<double> tmp(n);
+        if (write)
+        {
+            for (size_t i = 0; i < n; ++i)
+                tmp[i] = cm1 ? h[i] - 1.0 : h[i];
+            return transfer_scalar_values<T>(b, b->C, tmp.data(), true);
+        }
+        sphx_status st = transfer_scalar_values<T>(b, b->C, tmp.data(), false);
+        for (size_t i = 0; i < n; ++i)
+            h[i] = cm1 ? tmp[i] + 1.0 : tmp[i];
+        return st;
+    }
+    case SPHX_FIELD_COMPRESSION_RATE:
+        return scalar_dev(b->dC);
+    case SPHX_FIELD_VOLUME_REF:
+        return scalar_dev(b->Vol_ref);
+    case SPHX_FIELD_COMPRESSION_SUM:
+        return scalar_dev(b->S);
+    case SPHX_FIELD_INTERPOLATED:
+        return scalar_dev(b->interp);
+    case SPHX_FIELD_FORCE:
+        return soa_dev(b->force, T(0));
+    case SPHX_FIELD_FORCE_PRIOR:
+        return soa_dev(b->force_prior, T(0));
+    case SPHX_FIELD_NORMAL:
+        return soa_dev(b->normal, T(0));
+    case SPHX_FIELD_WALL_VELOCITY:
+        return soa_dev(b->wall_vel, T(0));
+    case SPHX_FIELD_WALL_ACCELERATION:
+        return soa_dev(b->wall_acc, T(0));
+    case SPHX_FIELD_VELOCITY:
+    {
+        std::vector<Rec4<T>> tmp(n);
+        if (write)
+        {
+            for (size_t i = 0; i < n; ++i)
+            {
+                tmp[i].x = T(h[i * D]);
+                tmp[i].y = T(h[i * D + 1]);
+                tmp[i].z = D == 3 ? T(h[i * D + 2]) : T(0);
+                tmp[i].w = T(0);
+            }
+            SPHX_HIP(hipMemcpyAsync(b->vel, tmp.data(), n * sizeof(Rec4<T>), hipMemcpyHostToDevice, s));
+            SPHX_HIP(hipStreamSynchronize(s));
+        }
+        else
+        {
+            SPHX_HIP(hipMemcpyAsync(tmp.data(), b->vel, n * sizeof(Rec4<T>), hipMemcpyDeviceToHost, s));
+            SPHX_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i)
+            {
+                h[i * D] = (double)tmp[i].x;
+                h[i * D + 1] = (double)tmp[i].y;
+                if (D == 3)
+                    h[i * D + 2] = (double)tmp[i].z;
+            }
+        }
+        return SPHX_OK;
+    }
+    case SPHX_FIELD_ORIGINAL_ID:
+        return u32_dev(b->original_id);
+    case SPHX_FIELD_SORTED_ID:
+        return u32_dev(b->sorted_id);
+    case SPHX_FIELD_SORT_SEQUENCE:
+        return u32_dev(b->sequence);
+    default:
+        return arg_error("unknown field id");
+    }
+}
+} // namespace
+
+} // namespace sphx
+
+using namespace sphx;
+
+namespace
+{
+bool relation_ready(sphx_relation *r)
+{
+    return r && r->built && r->built_version == r->source->config_version &&
+           r->target_version == r->target->config_version;
+}
+sphx_status check_fluid_relations(sphx_relation *inner, sphx_relation *const *walls, int n_wall, bool need_built)
+{
+    if (!inner || !inner->inner)
+        return arg_error("an inner relation of the fluid body is required");
+    if (n_wall < 0 || n_wall > 4 || (n_wall > 0 && !walls))
+        return arg_error("0..4 contact relations are supported");
+    if (inner->source->desc.kind != SPHX_BODY_FLUID)
+        return arg_error("the inner relation's body is not a fluid body");
+    if (need_built && !relation_ready(inner))
+        return state_error("inner relation not updated since the last configuration change");
+    for (int k = 0; k < n_wall; ++k)
+    {
+        sphx_relation *w = walls[k];
+        if (!w || w->inner || w->source != inner->source)
+            return arg_error("contact relations must start from the same fluid body");
+        if (need_built && !relation_ready(w))
+            return state_error("contact relation not updated since the last configuration change");
+    }
+    return SPHX_OK;
+}
+} // namespace
+
+extern "C" {
+
+const char *sphx_backend_name(void) { return "hip-gfx950"; }
+const char *sphx_last_error(void) { return g_error.c_str(); }
+
+sphx_status sphx_context_create(int device, sphx_context **out)
+{
+    if (!out)
+        return arg_error("null out");
+    int count = 0;
+    SPHX_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return arg_error("device index out of range");
+    SPHX_HIP(hipSetDevice(device));
+    std::unique_ptr<sphx_context> ctx(new sphx_context());
+    ctx->device = device;
+    SPHX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    SPHX_HIP(hipMalloc(&ctx->scalars_d, kScalarSlots * sizeof(unsigned long long)));
+    SPHX_HIP(hipHostMalloc(&ctx->scalars_h, kScalarSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    std::memset(ctx->scalars_h, 0, kScalarSlots * sizeof(unsigned long long));
+    SPHX_HIP(hipMemcpy(ctx->scalars_d, ctx->scalars_h, kScalarSlots * sizeof(unsigned long long),
+                       hipMemcpyHostToDevice));
+    *out = ctx.release();
+    return SPHX_OK;
+}
+
+sphx_status sphx_context_destroy(sphx_context *ctx)
+{
+    if (!ctx)
+        return SPHX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->scalars_d);
+    (void)hipHostFree(ctx->scalars_h);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SPHX_OK;
+}
+
+sphx_status sphx_context_synchronize(sphx_context *ctx)
+{
+    if (!ctx)
+        return arg_error("null context");
+    SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    return SPHX_OK;
+}
+
+sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_body **out)
+{
+    if (!ctx || !d || !out)
+        return arg_error("null argument");
+    if (d->dim != 2 && d->dim != 3)
+        return arg_error("dim must be 2 or 3");
+    if (d->precision != SPHX_FP64 && d->precision != SPHX_MIXED)
+        return arg_error("precision must be SPHX_FP64 or SPHX_MIXED");
+    if (!(d->spacing_ref > 0) || !(d->h_spacing_ratio > 0))
+        return arg_error("spacing_ref and h_spacing_ratio must be positive");
+    if (d->kind == SPHX_BODY_FLUID && (!(d->rho0 > 0) || !(d->c0 > 0)))
+        return arg_error("fluid bodies need positive rho0 and c0");
+    SPHX_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<sphx_body> b(new sphx_body());
+    b->ctx = ctx;
+    b->desc = *d;
+    b->D = d->dim;
+    b->P = d->precision;
+    b->n = d->n_particles;
+    // SPHAdaptation: h = h_spacing_ratio * (resolution / refinement), adaptation.cpp:12-19
+    b->h = d->h_spacing_ratio * (d->spacing_ref / 1.0);
+    const size_t n = b->n;
+    const size_t T = b->P == P_FP64 ? sizeof(double) : sizeof(float);
+    const size_t R4 = b->P == P_FP64 ? sizeof(Rec4<double>) : sizeof(Rec4<float>);
+    b->tsize = T;
+    sphx_status s;
+    for (int k = 0; k < 3; ++k)
+    {
+        if ((s = alloc(b.get(), (void **)&b->pos[k], n * sizeof(double))) ||
+            (s = alloc(b.get(), (void **)&b->dpos[k], n * sizeof(double))) ||
+            (s = alloc(b.get(), &b->force[k], n * T)) || (s = alloc(b.get(), &b->force_prior[k], n * T)) ||
+            (s = alloc(b.get(), &b->normal[k], n * T)) || (s = alloc(b.get(), &b->wall_vel[k], n * T)) ||
+            (s = alloc(b.get(), &b->wall_acc[k], n * T)))
+            return s;
+    }
+    void **scalars[] = {&b->Vol, &b->mass, &b->rho, &b->p, &b->C, &b->dC, &b->Vol_ref, &b->S, &b->interp};
+    for (void **f : scalars)
+        if ((s = alloc(b.get(), f, n * T)))
+            return s;
+    if ((s = alloc(b.get(), &b->posvol, n * R4)) || (s = alloc(b.get(), &b->vel, n * R4)) ||
+        (s = alloc(b.get(), (void **)&b->original_id, n * 4)) || (s = alloc(b.get(), (void **)&b->sorted_id, n * 4)) ||
+        (s = alloc(b.get(), (void **)&b->sequence, n * 4)) || (s = alloc(b.get(), (void **)&b->perm, n * 4)))
+        return s;
+    // zero every field, then the reference's initial values
+    for (int k = 0; k < 3; ++k)
+    {
+        SPHX_HIP(hipMemsetAsync(b->pos[k], 0, n * sizeof(double), ctx->stream));
+        SPHX_HIP(hipMemsetAsync(b->dpos[k], 0, n * sizeof(double), ctx->stream));
+        for (void *f : {b->force[k], b->force_prior[k], b->normal[k], b->wall_vel[k], b->wall_acc[k]})
+            SPHX_HIP(hipMemsetAsync(f, 0, n * T, ctx->stream));
+    }
+    for (void **f : scalars)
+        SPHX_HIP(hipMemsetAsync(*f, 0, n * T, ctx->stream));
+    SPHX_HIP(hipMemsetAsync(b->posvol, 0, n * R4, ctx->stream));
+    SPHX_HIP(hipMemsetAsync(b->vel, 0, n * R4, ctx->stream));
+    SPHX_HIP(hipMemsetAsync(b->sequence, 0, n * 4, ctx->stream));
+    if (b->P == P_FP64)
+    {
+        if ((s = fill<double>(ctx, b->rho, n, d->rho0)) || (s = fill<double>(ctx, b->C, n, 1.0)) ||
+            (s = fill<double>(ctx, b->S, n, 1.0)))
+            return s;
+    }
+    else
+    {
+        // MIXED stores C - 1 (= 0 for C = 1)
+        if ((s = fill<float>(ctx, b->rho, n, (float)d->rho0)) || (s = fill<float>(ctx, b->S, n, 1.0f)))
+            return s;
+    }
+    if ((s = fill_iota(ctx, b->original_id, n)) || (s = fill_iota(ctx, b->sorted_id, n)))
+        return s;
+    SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    *out = b.release();
+    return SPHX_OK;
+}
+
+sphx_status sphx_body_destroy(sphx_body *b)
+{
+    if (!b)
+        return SPHX_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (void *p : b->allocations)
+        (void)hipFree(p);
+    delete b;
+    return SPHX_OK;
+}
+
+sphx_status sphx_body_smoothing_length(const sphx_body *b, double *h)
+{
+    if (!b || !h)
+        return arg_error("null argument");
+    *h = b->h;
+    return SPHX_OK;
+}
+
+static sphx_status body_rw(sphx_body *b, int field, void *host, size_t n, bool write)
+{
+    if (!b || !host)
+        return arg_error("null argument");
+    if (n != b->n)
+        return arg_error("particle count mismatch");
+    if (field < 0 || field >= SPHX_FIELD_COUNT)
+        return arg_error("unknown field id");
+    if (b->n == 0)
+        return SPHX_OK;
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_status s = b->P == P_FP64 ? transfer<double>(b, field, host, write) : transfer<float>(b, field, host, write);
+    if (s || !write)
+        return s;
+    if (field == SPHX_FIELD_VOLUME && !b->vol_written)
+    {
+        // first write: VolumetricMeasureRef = Vol (density_regularization.hpp:18-20),
+        // Mass = rho0 * Vol (materials/base_material.cpp:37-40)
+        b->vol_written = true;
+        const double *v = (const double *)host;
+        std::vector<double> m(b->n);
+        for (size_t i = 0; i < b->n; ++i)
+            m[i] = b->desc.rho0 * v[i];
+        if ((s = body_rw(b, SPHX_FIELD_VOLUME_REF, host, n, true)) ||
+            (s = body_rw(b, SPHX_FIELD_MASS, m.data(), n, true)))
+            return s;
+    }
+    if (field == SPHX_FIELD_POSITION || field == SPHX_FIELD_VOLUME)
+    {
+        if (field == SPHX_FIELD_POSITION)
+            b->config_version++;
+        if ((s = refresh_posvol(b)))
+            return s;
+        SPHX_HIP(hipStreamSynchronize(b->ctx->stream));
+    }
+    return SPHX_OK;
+}
+
+sphx_status sphx_body_write(sphx_body *b, int field, const void *host, size_t n)
+{
+    return body_rw(b, field, const_cast<void *>(host), n, true);
+}
+sphx_status sphx_body_read(sphx_body *b, int field, void *host, size_t n)
+{
+    return body_rw(b, field, host, n, false);
+}
+
+sphx_status sphx_cell_linked_list_create(sphx_body *b, const double *lo, const double *up, double gs,
+                                         uint32_t buffer_width, sphx_cell_linked_list **out)
+{
+    if (!b || !lo || !up || !out || !(gs > 0))
+        return arg_error("bad cell linked list arguments");
+    std::unique_ptr<sphx_cell_linked_list> c(new sphx_cell_linked_list());
+    c->body = b;
+    // Mesh(tentative_bounds, grid_spacing, buffer_width), base_mesh.cpp:6-16
+    MeshParams &m = c->mesh;
+    m.gs = gs;
+    uint64_t ncells = 1;
+    const double buffer = double(buffer_width) * gs;
+    for (int k = 0; k < 3; ++k)
+    {
+        if (k < b->D)
+        {
+            m.lower[k] = lo[k] - buffer;
+            double dim = up[k] + buffer - m.lower[k];
+            double g = std::ceil(dim / gs);
+            if (!(g >= 1.0) || g > 1.0e9)
+                return arg_error("cell linked list bounds/grid spacing give an invalid mesh");
+            m.all_grid_points[k] = (int)g + 1;
+            m.all_cells[k] = m.all_grid_points[k] - 1;
+            ncells *= (uint64_t)m.all_cells[k];
+        }
+        else
+        {
+            m.lower[k] = 0;
+            m.all_grid_points[k] = 2;
+            m.all_cells[k] = 1;
+        }
+    }
+    if (ncells >= (1ull << 31))
+        return arg_error("cell linked list has more than 2^31 cells");
+    c->ncells = ncells;
+    int bits = 0;
+    while (bits < 32 && ((ncells - 1) >> bits) != 0)
+        ++bits;
+    c->key_bits = bits;
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    SPHX_HIP(hipMalloc(&c->cell_offset, (ncells + 1) * sizeof(uint32_t)));
+    SPHX_HIP(hipMalloc(&c->particle_index, std::max<size_t>(b->n, 1) * sizeof(uint32_t)));
+    SPHX_HIP(hipMalloc(&c->keys, std::max<size_t>(b->n, 1) * sizeof(uint32_t)));
+    *out = c.release();
+    return SPHX_OK;
+}
+
+sphx_status sphx_cell_linked_list_destroy(sphx_cell_linked_list *c)
+{
+    if (!c)
+        return SPHX_OK;
+    (void)hipStreamSynchronize(c->body->ctx->stream);
+    (void)hipFree(c->cell_offset);
+    (void)hipFree(c->particle_index);
+    (void)hipFree(c->keys);
+    delete c;
+    return SPHX_OK;
+}
+
+sphx_status sphx_cell_linked_list_mesh(const sphx_cell_linked_list *c, int *all_cells, double *lower, double *gs)
+{
+    if (!c)
+        return arg_error("null cell linked list");
+    for (int k = 0; k < c->body->D; ++k)
+    {
+        if (all_cells)
+            all_cells[k] = c->mesh.all_cells[k];
+        if (lower)
+            lower[k] = c->mesh.lower[k];
+    }
+    if (gs)
+        *gs = c->mesh.gs;
+    return SPHX_OK;
+}
+
+sphx_status sphx_update_cell_linked_list(sphx_cell_linked_list *c)
+{
+    if (!c)
+        return arg_error("null cell linked list");
+    SPHX_HIP(hipSetDevice(c->body->ctx->device));
+    sphx_status s = cell_linked_list_update(c);
+    if (s)
+        return s;
+    c->built_version = c->body->config_version;
+    return SPHX_OK;
+}
+
+sphx_status sphx_cell_linked_list_read(sphx_cell_linked_list *c, uint32_t *cell_offset, size_t n_off,
+                                       uint32_t *particle_index, size_t n_idx)
+{
+    if (!c)
+        return arg_error("null cell linked list");
+    if (!c->built || c->built_version != c->body->config_version)
+        return state_error("cell linked list not updated since the last configuration change");
+    hipStream_t s = c->body->ctx->stream;
+    if (cell_offset)
+    {
+        if (n_off != c->ncells + 1)
+        {
+            set_error("cell_offset size must be n_cells + 1");
+            return SPHX_E_CAPACITY;
+        }
+        SPHX_HIP(hipMemcpyAsync(cell_offset, c->cell_offset, n_off * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (particle_index)
+    {
+        if (n_idx != c->body->n)
+        {
+            set_error("particle_index size must be n_particles");
+            return SPHX_E_CAPACITY;
+        }
+        if (n_idx)
+            SPHX_HIP(hipMemcpyAsync(particle_index, c->particle_index, n_idx * 4, hipMemcpyDeviceToHost, s));
+    }
+    SPHX_HIP(hipStreamSynchronize(s));
+    return SPHX_OK;
+}
+
+static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_linked_list *cll, bool inner,
+                                   sphx_relation **out)
+{
+    std::unique_ptr<sphx_relation> r(new sphx_relation());
+    r->inner = inner;
+    r->source = src;
+    r->target = tar;
+    r->cll = cll;
+    // Neighbor<SPHAdaptation,SPHAdaptation>: src_inv_h = 1/h_src, inv_h = 1/max(h_src, h_tar)
+    r->src_inv_h = 1.0 / src->h;
+    r->inv_h = 1.0 / std::max(src->h, tar->h);
+    SPHX_HIP(hipSetDevice(src->ctx->device));
+    SPHX_HIP(hipMalloc(&r->neighbor_size, (size_t)(src->n + 1) * sizeof(uint32_t)));
+    SPHX_HIP(hipMalloc(&r->particle_offset, (size_t)(src->n + 1) * sizeof(uint32_t)));
+    SPHX_HIP(hipMemsetAsync(r->neighbor_size, 0, (size_t)(src->n + 1) * sizeof(uint32_t), src->ctx->stream));
+    sphx_status s = build_kernel_poly(r.get());
+    if (s)
+        return s;
+    *out = r.release();
+    return SPHX_OK;
+}
+
+sphx_status sphx_relation_create_inner(sphx_body *b, sphx_cell_linked_list *cll, sphx_relation **out)
+{
+    if (!b || !cll || !out || cll->body != b)
+        return arg_error("inner relation needs the body's own cell linked list");
+    return relation_create(b, b, cll, true, out);
+}
+
+sphx_status sphx_relation_create_contact(sphx_body *src, sphx_body *tar, sphx_cell_linked_list *cll,
+                                         sphx_relation **out)
+{
+    if (!src || !tar || !cll || !out || cll->body != tar)
+        return arg_error("contact relation needs the target body's cell linked list");
+    if (src->D != tar->D || src->P != tar->P || src->ctx != tar->ctx)
+        return arg_error("contact relation bodies differ in dimension, precision or context");
+    return relation_create(src, tar, cll, false, out);
+}
+
+sphx_status sphx_relation_destroy(sphx_relation *r)
+{
+    if (!r)
+        return SPHX_OK;
+    (void)hipStreamSynchronize(r->source->ctx->stream);
+    (void)hipFree(r->neighbor_size);
+    (void)hipFree(r->particle_offset);
+    if (r->neighbor_index)
+        (void)hipFree(r->neighbor_index);
+    if (r->poly)
+        (void)hipFree(r->poly);
+    delete r;
+    return SPHX_OK;
+}
+
+sphx_status sphx_update_relation(sphx_relation *r)
+{
+    if (!r)
+        return arg_error("null relation");
+    sphx_cell_linked_list *c = r->cll;
+    if (!c->built || c->built_version != c->body->config_version)
+        return state_error("target cell linked list not updated since the last configuration change");
+    sphx_context *ctx = r->source->ctx;
+    SPHX_HIP(hipSetDevice(ctx->device));
+    sphx_status s;
+    // count -> exclusive scan over n+1 (last count is 0) -> grow -> fill
+    if ((s = relation_count(r)))
+        return s;
+    uint32_t total = 0;
+    if ((s = exclusive_scan_u32(ctx, r->neighbor_size, r->particle_offset, (size_t)r->source->n + 1, &total)))
+        return s;
+    if (total > r->capacity || !r->neighbor_index)
+    {
+        // reference grows NeighborIndex on overflow (update_body_relation.hpp:145-155)
+        SPHX_HIP(hipStreamSynchronize(ctx->stream));
+        if (r->neighbor_index)
+            SPHX_HIP(hipFree(r->neighbor_index));
+        r->neighbor_index = nullptr;
+        size_t cap = std::max<size_t>((size_t)total + total / 4, 64);
+        SPHX_HIP(hipMalloc(&r->neighbor_index, cap * sizeof(uint32_t)));
+        r->capacity = cap;
+    }
+    r->total = total;
+    if ((s = relation_fill(r)))
+        return s;
+    r->built = true;
+    r->built_version = r->source->config_version;
+    r->target_version = r->target->config_version;
+    return SPHX_OK;
+}
+
+sphx_status sphx_relation_read(sphx_relation *r, uint32_t *size, uint32_t *offset, uint32_t *index, size_t capacity,
+                               size_t *total)
+{
+    if (!r)
+        return arg_error("null relation");
+    if (!relation_ready(r))
+        return state_error("relation not updated since the last configuration change");
+    hipStream_t s = r->source->ctx->stream;
+    size_t n = r->source->n;
+    if (total)
+        *total = r->total;
+    if (size && n)
+        SPHX_HIP(hipMemcpyAsync(size, r->neighbor_size, n * 4, hipMemcpyDeviceToHost, s));
+    if (offset)
+        SPHX_HIP(hipMemcpyAsync(offset, r->particle_offset, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    if (index)
+    {
+        if (capacity < r->total)
+        {
+            set_error("neighbor_index capacity too small");
+            return SPHX_E_CAPACITY;
+        }
+        if (r->total)
+            SPHX_HIP(hipMemcpyAsync(index, r->neighbor_index, r->total * 4, hipMemcpyDeviceToHost, s));
+    }
+    SPHX_HIP(hipStreamSynchronize(s));
+    return SPHX_OK;
+}
+
+sphx_status sphx_particle_sort(sphx_body *b, sphx_cell_linked_list *c)
+{
+    if (!b || !c || c->body != b)
+        return arg_error("sort needs the body's own cell linked list");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_status s;
+    if ((s = morton_sequence(b, c)))
+        return s;
+    // UpdateSortableVariables over the evolving variables only
+    for (int k = 0; k < b->D; ++k)
+        if ((s = gather_double(b, &b->pos[k], b->perm)) || (s = gather_real(b, &b->force_prior[k], b->perm)))
+            return s;
+    if ((s = gather_real(b, &b->Vol, b->perm)) || (s = gather_u32(b->ctx, b->original_id, b->perm, b->n)) ||
+        (s = gather_rec4(b, &b->vel, b->perm)) || (s = gather_real(b, &b->mass, b->perm)) ||
+        (s = gather_real(b, &b->C, b->perm)) || (s = gather_real(b, &b->dC, b->perm)) ||
+        (s = gather_real(b, &b->Vol_ref, b->perm)) || (s = sorted_id_update(b)) || (s = refresh_posvol(b)))
+        return s;
+    b->config_version++;
+    return SPHX_OK;
+}
+
+sphx_status sphx_gravity_force(sphx_body *b, const double *g)
+{
+    if (!b || !g)
+        return arg_error("null argument");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    return gravity_force(b, g);
+}
+
+sphx_status sphx_density_regularization(sphx_body *b, int flow)
+{
+    if (!b || (flow != SPHX_FLOW_INTERNAL && flow != SPHX_FLOW_FREE_SURFACE))
+        return arg_error("bad density regularization arguments");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    return density_regularization(b, flow);
+}
+
+sphx_status sphx_advection_step_setup(sphx_body *b)
+{
+    if (!b)
+        return arg_error("null body");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    return advection_step_setup(b);
+}
+
+sphx_status sphx_update_particle_position(sphx_body *b)
+{
+    if (!b)
+        return arg_error("null body");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_status s = update_position(b);
+    if (!s)
+        b->config_version++;
+    return s;
+}
+
+sphx_status sphx_compression_summation(sphx_relation *inner, sphx_relation *const *contact, int n)
+{
+    sphx_status s = check_fluid_relations(inner, contact, n, true);
+    if (s)
+        return s;
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return compression_summation(inner, contact, n);
+}
+
+sphx_status sphx_acoustic_step_1st_half(sphx_relation *inner, sphx_relation *const *walls, int n, int riemann,
+                                        double dt)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n, true);
+    if (s)
+        return s;
+    if (riemann < SPHX_RIEMANN_ACOUSTIC || riemann > SPHX_RIEMANN_DISSIPATIVE)
+        return arg_error("unknown Riemann solver");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return acoustic_1st_half(inner, walls, n, riemann, dt, false);
+}
+
+sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *const *walls, int n, int riemann,
+                                        double dt)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n, true);
+    if (s)
+        return s;
+    if (riemann < SPHX_RIEMANN_ACOUSTIC || riemann > SPHX_RIEMANN_DISSIPATIVE)
+        return arg_error("unknown Riemann solver");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return acoustic_2nd_half(inner, walls, n, riemann, dt, true, false);
+}
+
+static sphx_status read_max(sphx_context *ctx, double *out)
+{
+    sphx_status s = read_scalars(ctx, SLOT_MAX_A, 1);
+    if (s)
+        return s;
+    *out = from_ordered_bits(ctx->scalars_h[SLOT_MAX_A]);
+    return SPHX_OK;
+}
+
+sphx_status sphx_acoustic_time_step(sphx_body *b, double cfl, double *dt)
+{
+    if (!b || !dt)
+        return arg_error("null argument");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_context *ctx = b->ctx;
+    sphx_status s;
+    if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
+        (s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX_A)))
+        return s;
+    double m = 0;
+    if ((s = read_max(ctx, &m)))
+        return s;
+    // AcousticTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.hpp:30-36
+    *dt = cfl * b->h / (m + 2.71051e-20);
+    return SPHX_OK;
+}
+
+sphx_status sphx_advection_time_step(sphx_body *b, double u_ref, double cfl, double *dt)
+{
+    if (!b || !dt)
+        return arg_error("null argument");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_context *ctx = b->ctx;
+    sphx_status s;
+    if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
+        (s = advection_speed_max(b, ctx->scalars_d + SLOT_MAX_A)))
+        return s;
+    double m = 0;
+    if ((s = read_max(ctx, &m)))
+        return s;
+    // AdvectionTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.cpp:23-26
+    *dt = cfl * b->h / (std::max(std::sqrt(m), u_ref) + 2.71051e-20);
+    return SPHX_OK;
+}
+
+sphx_status sphx_total_mechanical_energy(sphx_body *b, const double *g, double *e)
+{
+    if (!b || !g || !e)
+        return arg_error("null argument");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_context *ctx = b->ctx;
+    sphx_status s = mechanical_energy(b, g, (double *)(ctx->scalars_d + SLOT_SUM_BASE));
+    if (s || (s = read_scalars(ctx, SLOT_SUM_BASE, 1)))
+        return s;
+    std::memcpy(e, ctx->scalars_h + SLOT_SUM_BASE, sizeof(double));
+    return SPHX_OK;
+}
+
+sphx_status sphx_upper_front_in_axis(sphx_body *b, int axis, double *front)
+{
+    if (!b || !front || axis < 0 || axis >= b->D)
+        return arg_error("bad upper front arguments");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    sphx_context *ctx = b->ctx;
+    sphx_status s;
+    if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
+        (s = axis_max(b, axis, ctx->scalars_d + SLOT_MAX_A)))
+        return s;
+    return read_max(ctx, front);
+}
+
+sphx_status sphx_interpolate_contact(sphx_relation *r, int field, double *out, size_t n_obs)
+{
+    if (!r || r->inner || !out)
+        return arg_error("interpolation needs a contact relation");
+    if (!relation_ready(r))
+        return state_error("observer relation not updated since the last configuration change");
+    if (n_obs != r->source->n)
+        return arg_error("observer count mismatch");
+    if (field != SPHX_FIELD_PRESSURE && field != SPHX_FIELD_DENSITY && field != SPHX_FIELD_COMPRESSION)
+        return arg_error("unsupported interpolation field");
+    SPHX_HIP(hipSetDevice(r->source->ctx->device));
+    sphx_status s = interpolate_contact(r, field);
+    if (s)
+        return s;
+    if ((s = sphx_body_read(r->source, SPHX_FIELD_INTERPOLATED, out, n_obs)))
+        return s;
+    if (field == SPHX_FIELD_COMPRESSION && r->source->P == P_MIXED)
+        for (size_t i = 0; i < n_obs; ++i)
+            out[i] += 1.0; // MIXED stores C - 1
+    return SPHX_OK;
+}
+
+sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                                 sphx_cell_linked_list *cll, const sphx_advection_config *cfg, double *time,
+                                 uint32_t *steps)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n_wall, true);
+    if (s)
+        return s;
+    if (!cfg || !time || !cll || cll != inner->cll)
+        return arg_error("advection step needs the fluid cell linked list, a config and a time pointer");
+    sphx_body *b = inner->source;
+    sphx_context *ctx = b->ctx;
+    SPHX_HIP(hipSetDevice(ctx->device));
+    if ((s = compression_summation(inner, walls, n_wall)) || (s = density_regularization(b, cfg->flow_type)) ||
+        (s = advection_step_setup(b)))
+        return s;
+    double adv_dt = 0.0;
+    if ((s = sphx_advection_time_step(b, cfg->u_ref, cfg->advection_cfl, &adv_dt)))
+        return s;
+    // AcousticTimeStepCK of the first sub-step from the current state
+    double dt = 0.0;
+    if ((s = sphx_acoustic_time_step(b, cfg->acoustic_cfl, &dt)))
+        return s;
+    double relaxation_time = 0.0;
+    uint32_t count = 0;
+    while (relaxation_time < adv_dt)
+    {
+        if ((s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, dt, true)) ||
+            (s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
+            (s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, dt, false, true)))
+            return s;
+        relaxation_time += dt;
+        *time += dt;
+        ++count;
+        double m = 0.0;
+        if ((s = read_max(ctx, &m)))
+            return s;
+        dt = cfg->acoustic_cfl * b->h / (m + 2.71051e-20);
+        if (!std::isfinite(dt) || !(dt > 0.0))
+        {
+            set_error("non-finite acoustic time step: the fluid state diverged");
+            return SPHX_E_NONFINITE;
+        }
+    }
+    if ((s = sphx_update_particle_position(b)))
+        return s;
+    if (cfg->sort_now && (s = sphx_particle_sort(b, cll)))
+        return s;
+    if ((s = sphx_update_cell_linked_list(cll)) || (s = sphx_update_relation(inner)))
+        return s;
+    for (int k = 0; k < n_wall; ++k)
+        if ((s = sphx_update_relation(walls[k])))
+            return s;
+    if (steps)
+        *steps = count;
+    return SPHX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,147 @@
+// sphx_internal.hpp -- host-side objects behind the opaque handles of sphx.h
+// and the launch functions each kernel translation unit provides.
+#pragma once
+#include "../../include/sphx.h"
+#include "sphx_device.hpp"
+
+#include <string>
+#include <vector>
+
+namespace sphx
+{
+void set_error(const std::string &msg);
+sphx_status hip_fail(hipError_t e, const char *what);
+
+#define SPHX_HIP(call)                                \
+    do                                                \
+    {                                                 \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess)                         \
+            return ::sphx::hip_fail(e_, #call);       \
+    } while (0)
+
+// grow-only device scratch owned by a context
+struct Scratch
+{
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    sphx_status reserve(size_t n);
+    ~Scratch();
+};
+
+constexpr int kScalarSlots = 64;
+enum ScalarSlot
+{
+    SLOT_MAX_A = 0,      // ordered-bits max accumulators (u64)
+    SLOT_MAX_B = 1,
+    SLOT_TOTAL = 2,      // u32 totals returned by scans (stored as u64)
+    SLOT_DT = 3,         // double: acoustic dt of the current sub-step
+    SLOT_SUM_BASE = 8    // double partial sums region start (reduce_sum final)
+};
+} // namespace sphx
+
+struct sphx_context
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    unsigned long long *scalars_d = nullptr; // kScalarSlots u64 on the device
+    unsigned long long *scalars_h = nullptr; // pinned mirror
+    sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp;
+};
+
+struct sphx_body
+{
+    sphx_context *ctx = nullptr;
+    sphx_body_desc desc{};
+    int D = 3, P = 0;
+    uint32_t n = 0;
+    double h = 0.0;      // smoothing length (computed in double as the FP64 reference)
+    bool vol_written = false;
+    uint64_t config_version = 0; // bumped whenever positions or particle order change
+    size_t tsize = 8;    // sizeof(T)
+    double *pos[3] = {nullptr, nullptr, nullptr};
+    double *dpos[3] = {nullptr, nullptr, nullptr};
+    void *posvol = nullptr; // Rec4<T>
+    void *vel = nullptr;    // Rec4<T>
+    void *force[3] = {nullptr, nullptr, nullptr};
+    void *force_prior[3] = {nullptr, nullptr, nullptr};
+    void *normal[3] = {nullptr, nullptr, nullptr};
+    void *wall_vel[3] = {nullptr, nullptr, nullptr};
+    void *wall_acc[3] = {nullptr, nullptr, nullptr};
+    void *Vol = nullptr, *mass = nullptr, *rho = nullptr, *p = nullptr, *C = nullptr;
+    void *dC = nullptr, *Vol_ref = nullptr, *S = nullptr, *interp = nullptr;
+    uint32_t *original_id = nullptr, *sorted_id = nullptr, *sequence = nullptr, *perm = nullptr;
+    std::vector<void *> allocations;
+};
+
+struct sphx_cell_linked_list
+{
+    sphx_body *body = nullptr;
+    sphx::MeshParams mesh{};
+    uint64_t ncells = 0;
+    int key_bits = 0;
+    uint32_t *cell_offset = nullptr;    // ncells + 1
+    uint32_t *particle_index = nullptr; // n
+    uint32_t *keys = nullptr;           // n (sort keys scratch)
+    bool built = false;
+    uint64_t built_version = 0;         // body config_version the list was built for
+};
+
+struct sphx_relation
+{
+    bool inner = true;
+    sphx_body *source = nullptr, *target = nullptr;
+    sphx_cell_linked_list *cll = nullptr; // of the target
+    uint32_t *neighbor_size = nullptr;    // n_src
+    uint32_t *particle_offset = nullptr;  // n_src + 1
+    uint32_t *neighbor_index = nullptr;   // capacity
+    size_t capacity = 0, total = 0;
+    bool built = false;
+    uint64_t built_version = 0, target_version = 0; // config versions the lists were built for
+    double inv_h = 0, src_inv_h = 0;      // Neighbor<SPHAdaptation,SPHAdaptation> constants
+    void *poly = nullptr;                 // KernelPoly<T> on the device
+};
+
+namespace sphx
+{
+// ---- configuration kernels (config_kernels.hip) ----
+sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *out, size_t n,
+                               uint32_t *total_host);
+sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
+sphx_status cell_linked_list_update(sphx_cell_linked_list *cll);
+sphx_status relation_count(sphx_relation *rel);
+sphx_status relation_fill(sphx_relation *rel);
+sphx_status morton_sequence(sphx_body *b, sphx_cell_linked_list *cll);
+sphx_status gather_u32(sphx_context *ctx, uint32_t *field, const uint32_t *perm, size_t n);
+sphx_status gather_real(sphx_body *b, void **field, const uint32_t *perm);
+sphx_status gather_double(sphx_body *b, double **field, const uint32_t *perm);
+sphx_status gather_rec4(sphx_body *b, void **field, const uint32_t *perm);
+sphx_status sorted_id_update(sphx_body *b);
+sphx_status fill_iota(sphx_context *ctx, uint32_t *a, size_t n);
+
+// ---- fluid kernels (fluid_kernels.hip) ----
+sphx_status refresh_posvol(sphx_body *b);
+sphx_status gravity_force(sphx_body *b, const double *g);
+sphx_status density_regularization(sphx_body *b, int flow);
+sphx_status advection_step_setup(sphx_body *b);
+sphx_status update_position(sphx_body *b);
+sphx_status compression_summation(sphx_relation *inner, sphx_relation *const *c, int nc);
+// fuse_init_2nd: pass 1 also runs AcousticStep2ndHalf's initialize
+sphx_status acoustic_1st_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                              double dt, bool fuse_init_2nd);
+// run_init: launch the 2nd half's initialize (false when pass 1 fused it);
+// fuse_cfl: max-reduce the next sub-step's signal speed into SLOT_MAX_A
+sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                              double dt, bool run_init, bool fuse_cfl);
+sphx_status acoustic_signal_max(sphx_body *b, unsigned long long *slot);
+sphx_status advection_speed_max(sphx_body *b, unsigned long long *slot);
+sphx_status acoustic_dt_from_max(sphx_body *b, double cfl, const unsigned long long *slot, double *dt_out);
+sphx_status axis_max(sphx_body *b, int axis, unsigned long long *slot);
+sphx_status mechanical_energy(sphx_body *b, const double *g, double *out_device);
+sphx_status interpolate_contact(sphx_relation *rel, int field);
+sphx_status fill_u64(sphx_context *ctx, unsigned long long *p, size_t n, unsigned long long v);
+sphx_status build_kernel_poly(sphx_relation *rel);
+
+// ---- scalar read-back helper (sphx_api.hip) ----
+sphx_status read_scalars(sphx_context *ctx, int first, int count);
+} // namespace sphx
