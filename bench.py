@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""bench.py -- particle-steps/s of the 3D dam break (weakly-compressible SPH,
+CK dual-time loop) on one MI355X, FP32-mixed and plain FP64.
+
+Workload: tank 3.2 x 1.0 x 1.5 (z up), water column 1.2 x 1.0 x 0.6 at x = 0,
+dp = 0.00448 (8,008,376 fluid particles), 3 wall layers, rho0 = 1000,
+g = 9.81, c0 = 10 * 2 sqrt(g 0.6), AcousticRiemannSolverCK, free-surface
+density regularization.  One "step" is one advection step of the CK 3D
+dam-break loop: compression summation, regularization, setup, advection dt,
+the acoustic sub-steps (dt, 1st half, 2nd half) until they cover the
+advection dt, position update, cell linked list and relations (and the
+particle sort on every 100th step).  value = fluid particles x acoustic
+sub-steps / wall time of the K timed steps, state resident in HBM.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP32_VALU_PEAK = 157.3e12      # MI355X vector FP32 (AMD figure, packed issue)
+FP64_VALU_PEAK = 78.6e12       # MI355X vector FP64
+METRIC = "particle-steps/s, 3D dam break FP32-mixed vs FP64; p50/p99 scene latency"
+
+# Algorithmic HBM bytes per fluid particle (3D), counting each per-particle
+# field once per pass and neighbour data as cache-resident:
+#  pass 2 (AcousticStep2ndHalf interact + update + fused CFL signal speed):
+#    reads (x,y,z,Vol) record, v, C, dC, F_prior, m; writes dC, F, C, rho
+#  whole acoustic+advection step: the enumerated figure of the WCSPH step
+#    (two fused passes + amortised advection-level work, walls included).
+def _bytes(vec, scalar, rec):
+    """Per-particle algorithmic bytes of the two acoustic passes (3D)."""
+    pass1 = (rec + scalar * 4 + vec * 3) + (vec + scalar + vec) + 2 * 3 * 8
+    #       reads: (x,y,z,Vol), p, C, m, rho, F_prior, F, v | writes F, dC, v | dpos r+w (f64)
+    pass2 = (rec + vec + scalar * 2 + vec + scalar) + (scalar + vec + scalar + scalar)
+    #       reads: (x,y,z,Vol), v, C, dC, F_prior, m      | writes dC, F, C, rho
+    return pass1, pass2
+
+
+PASS1_BYTES = {"mixed": _bytes(12, 4, 16)[0], "fp64": _bytes(24, 8, 32)[0]}
+PASS2_BYTES = {"mixed": _bytes(12, 4, 16)[1], "fp64": _bytes(24, 8, 32)[1]}
+STEP_BYTES = {"mixed": 340.0, "fp64": 545.0}
+FLOP_PER_PAIR = 125.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(lib, geom, precision, warmup, steps):
+    from sphinxsys_amd import _capi as capi
+    from sphinxsys_amd import cases
+    t0 = time.time()
+    sc = cases.DamBreak(lib, geom, precision)
+    sc.initialize()
+    log(f"  scene ready in {time.time() - t0:.1f}s ({sc.fluid.n} fluid, {sc.wall.n} wall)")
+    for _ in range(warmup):
+        sc.advection_step()
+    sc.ctx.synchronize()
+    sc.ctx.profile(True)
+    a0, i0 = sc.acoustic_steps, sc.iterations
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sc.advection_step()
+    sc.ctx.synchronize()
+    wall = time.perf_counter() - t0
+    n_ac = sc.acoustic_steps - a0
+    prof = {k: sc.ctx.profile_read(k) for k in (
+        "acoustic_pass1", "acoustic_pass2", "acoustic_init", "compression_summation",
+        "relation_build", "cell_linked_list", "particle_sort",
+        "time_step_reduce", "state_update")}
+    sc.ctx.profile(False)
+    inner_size, _, _ = sc.inner.read()
+    wall_size, _, _ = sc.contact.read()
+    k_inner = float(inner_size.mean())
+    k_wall = float(wall_size.mean())
+    probe, energy = sc.observe()
+    front = sc.front.exec()
+    out = {"wall_s": wall, "acoustic_steps": n_ac, "advection_steps": sc.iterations - i0,
+           "n_fluid": sc.fluid.n, "n_wall": sc.wall.n, "profile": prof,
+           "mean_neighbors_inner": k_inner, "mean_neighbors_wall": k_wall,
+           "physical_time": sc.time, "probe_pressure": float(probe[0]), "energy": energy,
+           "front": front}
+    sc.close()
+    return out
+
+
+def run_cpu_baseline(geom, threads):
+    """The oracle (CPU restatement, FP64, std::thread pool) on the same 8M
+    workload: one full advection step after the initial configuration."""
+    from sphinxsys_amd import _capi as capi
+    from sphinxsys_amd import cases
+    from tests._util import oracle_lib
+    lib = oracle_lib()
+    lib.lib.sphx_oracle_set_threads(threads)
+    sc = cases.DamBreak(lib, geom, capi.FP64)
+    sc.initialize()
+    t0 = time.perf_counter()
+    n = sc.advection_step()
+    wall = time.perf_counter() - t0
+    sc.close()
+    return {"value": sc.fluid.n * n / wall, "wall_s": wall, "acoustic_steps": n}
+
+
+def copy_calibration(nbytes=1 << 30, repeats=3):
+    """Plain device-to-device copies of a known size through the HIP runtime:
+    the reference reading of the HBM counters (FETCH_SIZE / WRITE_SIZE)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(repeats):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {repeats} x {nbytes >> 20} MiB device-to-device copies")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30, help="timed advection steps")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed advection steps")
+    ap.add_argument("--dp", type=float, default=0.00448, help="particle spacing")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference kernel for HBM counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent scenes shard across GPUs without communication)", file=sys.stderr)
+        sys.exit(2)
+
+    import sphinxsys_amd
+    from sphinxsys_amd import _capi as capi
+    from sphinxsys_amd import cases
+
+    lib = sphinxsys_amd.load()
+    if args.copy_calibration:
+        copy_calibration()
+    t0 = time.time()
+    geom = cases.dambreak_3d(dp=args.dp)
+    log(f"geometry: {len(geom.fluid_pos)} fluid, {len(geom.wall_pos)} wall in {time.time() - t0:.1f}s")
+
+    log("FP32-mixed run")
+    mixed = run_gpu(lib, geom, capi.MIXED, args.warmup, args.steps)
+    fp64 = None
+    if not args.no_fp64:
+        log("FP64 run")
+        fp64 = run_gpu(lib, geom, capi.FP64, args.warmup, args.steps)
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline (oracle, {args.cpu_threads} threads)")
+        cpu = run_cpu_baseline(geom, args.cpu_threads)
+
+    n_f = mixed["n_fluid"]
+    value = n_f * mixed["acoustic_steps"] / mixed["wall_s"]
+    ms_per_step = 1000.0 * mixed["wall_s"] / args.steps
+
+    # dominant kernel: the acoustic pass with the larger total time
+    p1_ms, p1_n = mixed["profile"]["acoustic_pass1"]
+    p2_ms, p2_n = mixed["profile"]["acoustic_pass2"]
+    if p2_ms >= p1_ms:
+        dom, dom_ms, dom_n, dom_b = "acoustic_pass2", p2_ms, p2_n, PASS2_BYTES["mixed"]
+    else:
+        dom, dom_ms, dom_n, dom_b = "acoustic_pass1", p1_ms, p1_n, PASS1_BYTES["mixed"]
+    avg_ms = dom_ms / max(dom_n, 1)
+    achieved = dom_b * n_f / (avg_ms * 1e-3) / 1e9
+    k_bar = mixed["mean_neighbors_inner"] + mixed["mean_neighbors_wall"]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed (f32 pair math, f64 position/time)",
+        "data": "synthetic (lattice-generated dam-break geometry)",
+        "config": {
+            "workload": f"3D dam break, tank 3.2x1.0x1.5, column 1.2x1.0x0.6, dp={args.dp}, "
+                        f"{n_f} fluid + {mixed['n_wall']} wall particles, FP32-mixed",
+            "fluid_particles": n_f,
+            "wall_particles": mixed["n_wall"],
+            "acoustic_steps_timed": mixed["acoustic_steps"],
+            "mean_neighbors": k_bar,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_particle": dom_b,
+            "avg_launch_ms": avg_ms,
+        },
+        "step_roofline": {
+            "hbm_frac": STEP_BYTES["mixed"] * value / (HBM_PEAK_GBS * 1e9),
+            "valu_frac": FLOP_PER_PAIR * k_bar * value / FP32_VALU_PEAK,
+            "bytes_per_particle_step": STEP_BYTES["mixed"],
+            "flop_per_particle_step": FLOP_PER_PAIR * k_bar,
+        },
+        "stage_ms": {k: round(v[0], 3) for k, v in mixed["profile"].items()},
+        "observables": {"physical_time": mixed["physical_time"],
+                        "probe_pressure": mixed["probe_pressure"], "front": mixed["front"]},
+    }
+    if fp64:
+        v64 = n_f * fp64["acoustic_steps"] / fp64["wall_s"]
+        line["fp64"] = {"value": v64, "ms_per_step": 1000.0 * fp64["wall_s"] / args.steps,
+                        "acoustic_steps_timed": fp64["acoustic_steps"],
+                        "stage_ms": {k: round(v[0], 3) for k, v in fp64["profile"].items()},
+                        "probe_pressure": fp64["probe_pressure"], "front": fp64["front"]}
+        line["mixed_over_fp64"] = value / v64
+    if cpu:
+        line["cpu_baseline"] = {
+            "value": cpu["value"], "unit": "particle-steps/s", "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"one advection step ({cpu['acoustic_steps']} acoustic sub-steps + "
+                      f"configuration update) of the same 8M workload, FP64, "
+                      f"{cpu['wall_s']:.1f}s"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

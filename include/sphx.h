@@ -241,6 +241,18 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
                                 const sphx_advection_config *cfg, double *physical_time,
                                 uint32_t *acoustic_steps);
 
+/* ---- instrumentation ------------------------------------------------------------
+ * Kernel timing with HIP events recorded on the context's stream around each
+ * launch of the named stage (no reference counterpart; the reference times
+ * whole phases with TickCount, dambreak_sycl.cpp:175-259).  Stage names:
+ * "acoustic_pass1", "acoustic_pass2", "acoustic_init", "compression_summation",
+ * "relation_build", "cell_linked_list", "particle_sort",
+ * "time_step_reduce", "state_update".  Reading synchronises the stream and
+ * returns the accumulated milliseconds and launch count since enabling. */
+sphx_status sphx_profile_enable(sphx_context *ctx, int enable);
+sphx_status sphx_profile_read(sphx_context *ctx, const char *stage, double *total_ms,
+                              uint64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

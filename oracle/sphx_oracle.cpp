@@ -1010,6 +1010,22 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     return SPHX_OK;
 }
 
+// instrumentation entry points: the oracle records nothing
+sphx_status sphx_profile_enable(sphx_context *ctx, int)
+{
+    return ctx ? SPHX_OK : fail(SPHX_E_ARG, "null context");
+}
+sphx_status sphx_profile_read(sphx_context *ctx, const char *stage, double *ms, uint64_t *launches)
+{
+    if (!ctx || !stage)
+        return fail(SPHX_E_ARG, "null argument");
+    if (ms)
+        *ms = 0.0;
+    if (launches)
+        *launches = 0;
+    return SPHX_OK;
+}
+
 // oracle-only: thread count used by parallel_for (the CPU baseline reports it)
 int sphx_oracle_set_threads(int n)
 {

@@ -38,7 +38,8 @@ SYMBOLS = [
     "sphx_update_particle_position", "sphx_compression_summation",
     "sphx_acoustic_step_1st_half", "sphx_acoustic_step_2nd_half", "sphx_acoustic_time_step",
     "sphx_advection_time_step", "sphx_total_mechanical_energy", "sphx_upper_front_in_axis",
-    "sphx_interpolate_contact", "sphx_advection_step",
+    "sphx_interpolate_contact", "sphx_advection_step", "sphx_profile_enable",
+    "sphx_profile_read",
 ]
 
 
@@ -107,6 +108,8 @@ class SphxLibrary:
             "sphx_interpolate_contact": (C.c_int, [_vp, C.c_int, _dp, C.c_size_t]),
             "sphx_advection_step": (C.c_int, [_vp, _pp, C.c_int, _vp, C.POINTER(AdvectionConfig),
                                               _dp, _up]),
+            "sphx_profile_enable": (C.c_int, [_vp, C.c_int]),
+            "sphx_profile_read": (C.c_int, [_vp, C.c_char_p, _dp, C.POINTER(C.c_uint64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -32,6 +32,16 @@ class Context:
     def synchronize(self):
         self.lib.call("sphx_context_synchronize", self.h)
 
+    def profile(self, enable=True):
+        """Start (or stop) HIP-event timing of the library's kernel stages."""
+        self.lib.call("sphx_profile_enable", self.h, 1 if enable else 0)
+
+    def profile_read(self, stage):
+        """(total milliseconds, launches) of one stage since profile() was called."""
+        ms, n = C.c_double(), C.c_uint64()
+        self.lib.call("sphx_profile_read", self.h, stage.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
     def close(self):
         if self.h:
             self.lib.lib.sphx_context_destroy(self.h)

@@ -314,79 +314,325 @@ __global__ void k_iota_u32(uint32_t *a, size_t n)
 struct RelationArgs
 {
     PosPtrs src, tar;
+    // float pre-filter of the criterion: squared distance of the float copies
+    // below thr_lo is a neighbour, at or above thr_hi is not, in between the
+    // double criterion decides; valid for |x_i| <= pre_xmax on every axis
+    const Rec4<float> *src_f, *tar_f;
+    float thr_lo, thr_hi;
+    double pre_xmax;
     uint32_t n_src;
     MeshParams m;
     const uint32_t *cell_offset, *particle_index;
     double inv_h;
     int inner;
     uint32_t *neighbor_size;
-    const uint32_t *particle_offset;
-    uint32_t *neighbor_index;
+    const uint32_t *chunk_offset;
+    uint32_t *slice;
 };
 
-template <int D, bool FILL>
-__global__ void __launch_bounds__(kThreads) k_relation(RelationArgs a)
+// slices -> CSR NeighborIndex (the reference's layout), for reads
+__global__ void k_slice_to_csr(const uint32_t *count, const uint32_t *chunk, const uint32_t *slice,
+                               const uint32_t *offset, uint32_t *csr, uint32_t n)
 {
-    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
-    if (i >= a.n_src)
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
         return;
-    double xi[3];
-    load_pos<D>(a.src, i, xi);
-    int c[3], lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
-    cell_index<D>(a.m, xi, c);
+    const uint32_t base = chunk[i >> 6] + (i & 63u), o = offset[i];
+    for (uint32_t k = 0; k < count[i]; ++k)
+        csr[o + k] = slice[base + (k << 6)];
+}
+
+//---------------------------------------------------------------------------
+// staging sets: for each block of kStageBlock source particles, the sorted
+// distinct neighbours and the 16-bit rank of every neighbour entry in it.
+// A hash set in LDS collects the distinct indices, a bitonic sort fixes their
+// order (so the result does not depend on insertion order), a second hash
+// table maps an index to its rank.
+//---------------------------------------------------------------------------
+constexpr uint32_t kHashSize = 4096;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 2654435761u) >> 20; }
+
+__device__ __forceinline__ uint32_t hash_find(const uint32_t *ht, uint32_t key)
+{
+    uint32_t s = hash_slot(key);
+    while (ht[s] != key)
+        s = (s + 1u) & (kHashSize - 1u);
+    return s;
+}
+
+//---------------------------------------------------------------------------
+// one-pass relation build: a single candidate scan per source particle writes
+// the neighbour slices (fixed kmax rows per 64-particle chunk) and, for staged
+// relations, the block's distinct-neighbour set with the 16-bit ranks.
+// Rows of a wave are assembled in an LDS ring buffer and written out whole, so
+// the list writes are coalesced rows instead of scattered 4-byte stores.
+//---------------------------------------------------------------------------
+constexpr uint32_t kRing = 32;      // ring rows per wave
+constexpr uint32_t kFlushEvery = 8; // candidate iterations between flushes
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
 #pragma unroll
-    for (int k = 0; k < D; ++k)
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+struct BuildArgs
+{
+    RelationArgs r;
+    uint32_t kmax;
+    unsigned long long *needed; // max neighbour count seen when it exceeds kmax
+    uint32_t cap;
+    uint32_t *stage_count, *stage_keys;
+    uint16_t *local_slice;
+};
+
+template <int D, bool STAGE>
+__global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
+{
+    const RelationArgs &a = b.r;
+    // ring rows during the scan; reused for the sorted keys and the rank table after it
+    __shared__ uint32_t scratch[(kThreads / 64) * kRing * 64];
+    __shared__ uint32_t ht[STAGE ? kHashSize : 1];
+    __shared__ uint32_t hash_full;
+    static_assert(kStageMaxCap + kHashSize / 2 <= (kThreads / 64) * kRing * 64, "keys + rank fit in the ring");
+    uint32_t (*ring)[kRing][64] = reinterpret_cast<uint32_t (*)[kRing][64]>(scratch);
+    uint32_t *keys = scratch;                                          // after the scan
+    uint16_t *rank = reinterpret_cast<uint16_t *>(scratch + kStageMaxCap); // after the scan
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t blk = xcd_block_index();
+    const uint32_t i = blk * kThreads + t;
+    const bool valid = i < a.n_src;
+    if (STAGE)
     {
-        lo[k] = max(0, c[k] - 1);
-        hi[k] = min(a.m.all_cells[k], c[k] + 2);
+        for (uint32_t k = t; k < kHashSize; k += kThreads)
+            ht[k] = kEmpty;
+        if (t == 0)
+            hash_full = 0;
+        __syncthreads();
     }
-    uint32_t count = 0;
-    uint32_t w = FILL ? a.particle_offset[i] : 0u;
-    // with the last axis fastest, the cells of one row of the search box are
-    // consecutive, so their particles form one contiguous range
-    const int m_lo = D == 3 ? lo[1] : 0, m_hi = D == 3 ? hi[1] : 1;
-    for (int l = lo[0]; l < hi[0]; ++l)
-        for (int mm = m_lo; mm < m_hi; ++mm)
+    double xi[3] = {0.0, 0.0, 0.0};
+    int c[3] = {0, 0, 0};
+    Rec4<float> xf = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool exact_lane = false;
+    if (valid)
+    {
+        load_pos<D>(a.src, i, xi);
+        cell_index<D>(a.m, xi, c);
+        xf = a.src_f[i];
+        for (int k2 = 0; k2 < D; ++k2)
+            exact_lane |= !(fabs(xi[k2]) <= a.pre_xmax);
+    }
+    const size_t row0 = (size_t)(i >> 6) * b.kmax * 64u + lane;
+    uint32_t k = 0;       // neighbours found by this lane
+    uint32_t flushed = 0; // rows of this wave already written (wave-uniform)
+    auto flush = [&](bool last) {
+        const uint32_t mine = min(k, b.kmax);
+        const uint32_t kmax_w = wave_max_u32(valid ? mine : 0u);
+        const uint32_t kmin_w = wave_min_u32(valid ? mine : 0xffffffffu);
+        uint32_t target = last ? kmax_w : min(kmin_w, kmax_w);
+        // keep room for the next kFlushEvery appends of the fastest lane
+        if (kmax_w + kFlushEvery > flushed + kRing)
+            target = max(target, kmax_w + kFlushEvery - kRing);
+        for (uint32_t r = flushed; r < target; ++r)
+            a.slice[row0 + (size_t)r * 64u] = ring[wave][r % kRing][lane];
+        flushed = max(flushed, target);
+    };
+    // one candidate: float pre-filter, exact double criterion in the margin
+    // shell, append to the ring; returns the neighbour or kEmpty
+    auto test = [&](uint32_t j, const Rec4<float> &pf) -> uint32_t {
+        if (a.inner && j == i)
+            return kEmpty;
+        const float fx = xf.x - pf.x, fy = xf.y - pf.y, fz = D == 3 ? xf.z - pf.z : 0.0f;
+        float r2f = fx * fx + fy * fy;
+        if (D == 3)
+            r2f = r2f + fz * fz;
+        double r2 = r2f < a.thr_lo ? 0.0 : 8.0; // decided by the pre-filter
+        if (exact_lane || (r2f >= a.thr_lo && r2f < a.thr_hi))
         {
-            uint32_t first, last;
-            if (D == 2)
+            double xj[3];
+            load_pos<D>(a.tar, j, xj);
+            // NeighborCriterion: (inv_h * (x_i - x_j)).squaredNorm() < 4
+            const double s_0 = a.inv_h * (xi[0] - xj[0]);
+            const double s_1 = a.inv_h * (xi[1] - xj[1]);
+            r2 = s_0 * s_0 + s_1 * s_1;
+            if (D == 3)
             {
-                first = (uint32_t)(l * a.m.all_cells[1] + lo[1]);
-                last = (uint32_t)(l * a.m.all_cells[1] + hi[1]);
+                const double s_2 = a.inv_h * (xi[2] - xj[2]);
+                r2 = r2 + s_2 * s_2;
             }
+        }
+        if (!(r2 < 4.0))
+            return kEmpty;
+        if (k < b.kmax)
+        {
+            if (k < flushed)
+                a.slice[row0 + (size_t)k * 64u] = j;
             else
+                ring[wave][k % kRing][lane] = j;
+        }
+        ++k;
+        return j;
+    };
+    auto insert = [&](uint32_t found) {
+        // lanes of one cell scan the same candidates in lockstep: insert a key
+        // only where it differs from the previous lane's
+        const uint32_t prev = (uint32_t)__shfl_up((int)found, 1, 64);
+        if (found != kEmpty && (lane == 0 || prev != found))
+        {
+            uint32_t h = hash_slot(found), probes = 0;
+            while (true)
             {
-                first = (uint32_t)(l * a.m.all_cells[1] * a.m.all_cells[2] + mm * a.m.all_cells[2] + lo[2]);
-                last = first + (uint32_t)(hi[2] - lo[2]);
-            }
-            uint32_t s0 = a.cell_offset[first], s1 = a.cell_offset[last];
-            for (uint32_t s = s0; s < s1; ++s)
-            {
-                uint32_t j = a.particle_index[s];
-                if (a.inner && j == i)
-                    continue;
-                double xj[3];
-                load_pos<D>(a.tar, j, xj);
-                // NeighborCriterion: (inv_h * (x_i - x_j)).squaredNorm() < 4
-                double s_0 = a.inv_h * (xi[0] - xj[0]);
-                double s_1 = a.inv_h * (xi[1] - xj[1]);
-                double r2 = s_0 * s_0 + s_1 * s_1;
-                if (D == 3)
+                const uint32_t old = atomicCAS(&ht[h], kEmpty, found);
+                if (old == kEmpty || old == found)
+                    break;
+                h = (h + 1u) & (kHashSize - 1u);
+                if (++probes == kHashSize)
                 {
-                    double s_2 = a.inv_h * (xi[2] - xj[2]);
-                    r2 = r2 + s_2 * s_2;
-                }
-                if (r2 < 4.0)
-                {
-                    if (FILL)
-                        a.neighbor_index[w++] = j;
-                    else
-                        ++count;
+                    hash_full = 1;
+                    break;
                 }
             }
         }
-    if (!FILL)
-        a.neighbor_size[i] = count;
+    };
+    constexpr uint32_t kBatch = 4;
+    static_assert(kFlushEvery % kBatch == 0, "flush on batch boundaries");
+    // the 3x3 (3D) or 3 (2D) cell rows of the search box; the cells of one row
+    // are consecutive in the cell list, so their particles are one range
+    for (int dl = -1; dl <= 1; ++dl)
+        for (int dm = -1; dm <= (D == 3 ? 1 : -1); ++dm)
+        {
+            uint32_t s0 = 0, s1 = 0;
+            if (valid)
+            {
+                const int l = c[0] + dl;
+                if (D == 3)
+                {
+                    const int mm = c[1] + dm;
+                    if (l >= 0 && l < a.m.all_cells[0] && mm >= 0 && mm < a.m.all_cells[1])
+                    {
+                        const int lo2 = max(0, c[2] - 1), hi2 = min(a.m.all_cells[2], c[2] + 2);
+                        const uint32_t first =
+                            (uint32_t)(l * a.m.all_cells[1] * a.m.all_cells[2] + mm * a.m.all_cells[2] + lo2);
+                        s0 = a.cell_offset[first];
+                        s1 = a.cell_offset[first + (uint32_t)(hi2 - lo2)];
+                    }
+                }
+                else if (l >= 0 && l < a.m.all_cells[0])
+                {
+                    const int lo1 = max(0, c[1] - 1), hi1 = min(a.m.all_cells[1], c[1] + 2);
+                    s0 = a.cell_offset[(uint32_t)(l * a.m.all_cells[1] + lo1)];
+                    s1 = a.cell_offset[(uint32_t)(l * a.m.all_cells[1] + hi1)];
+                }
+            }
+            // wave-uniform trip count so that the ring can be flushed in the loop
+            const uint32_t trip = wave_max_u32(s1 - s0);
+            for (uint32_t it = 0; it < trip; it += kBatch)
+            {
+                uint32_t jj[kBatch];
+                Rec4<float> pf[kBatch];
+#pragma unroll
+                for (uint32_t q = 0; q < kBatch; ++q)
+                    jj[q] = s0 + it + q < s1 ? a.particle_index[s0 + it + q] : kEmpty;
+#pragma unroll
+                for (uint32_t q = 0; q < kBatch; ++q)
+                    if (jj[q] != kEmpty)
+                        pf[q] = a.tar_f[jj[q]];
+#pragma unroll
+                for (uint32_t q = 0; q < kBatch; ++q)
+                {
+                    const uint32_t found = jj[q] != kEmpty ? test(jj[q], pf[q]) : kEmpty;
+                    if (STAGE)
+                        insert(found);
+                }
+                if ((it + kBatch) % kFlushEvery == 0)
+                    flush(false);
+            }
+            flush(false);
+        }
+    flush(true);
+    if (valid)
+    {
+        a.neighbor_size[i] = min(k, b.kmax);
+        if (k > b.kmax)
+            atomicMax(b.needed, (unsigned long long)k);
+    }
+    if (!STAGE)
+        return;
+    //----- distinct set of the block, sorted, and the ranks of the entries
+    __syncthreads(); // every wave has flushed its ring: the scratch is free
+    constexpr uint32_t per = kHashSize / kThreads;
+    uint32_t occ = 0;
+    for (uint32_t q = 0; q < per; ++q)
+        occ += ht[t * per + q] != kEmpty;
+    uint32_t total;
+    uint32_t pos = block_exclusive_scan(occ, &total);
+    if (total > b.cap || hash_full)
+    {
+        if (t == 0)
+            b.stage_count[blk] = kStageOverflow;
+        return;
+    }
+    for (uint32_t q = 0; q < per; ++q)
+    {
+        const uint32_t v = ht[t * per + q];
+        if (v != kEmpty)
+            keys[pos++] = v;
+    }
+    uint32_t pow2 = 1;
+    while (pow2 < total)
+        pow2 <<= 1;
+    for (uint32_t q = total + t; q < pow2; q += kThreads)
+        keys[q] = kEmpty;
+    __syncthreads();
+    for (uint32_t size = 2; size <= pow2; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+        {
+            for (uint32_t q = t; q < pow2; q += kThreads)
+            {
+                const uint32_t partner = q ^ stride;
+                if (partner > q)
+                {
+                    const bool up = (q & size) == 0;
+                    const uint32_t x = keys[q], y = keys[partner];
+                    if ((x > y) == up)
+                    {
+                        keys[q] = y;
+                        keys[partner] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t r = t; r < total; r += kThreads)
+    {
+        rank[hash_find(ht, keys[r])] = (uint16_t)r;
+        b.stage_keys[(size_t)blk * b.cap + r] = keys[r];
+    }
+    __syncthreads();
+    // every entry of this lane was written by this lane (ring flush or direct)
+    const uint32_t mine = valid ? min(k, b.kmax) : 0u;
+    for (uint32_t q = 0; q < mine; ++q)
+        b.local_slice[row0 + (size_t)q * 64u] = rank[hash_find(ht, a.slice[row0 + (size_t)q * 64u])];
+    if (t == 0)
+        b.stage_count[blk] = total;
+}
+
+__global__ void k_chunk_stride(uint32_t *chunk, uint32_t nchunks, uint32_t stride)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= nchunks)
+        chunk[c] = c * stride;
 }
 
 //---------------------------------------------------------------------------
@@ -500,6 +746,7 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
 {
     sphx_body *b = cll->body;
     sphx_context *ctx = b->ctx;
+    ProfileScope prof(ctx, "cell_linked_list");
     const size_t nc = cll->ncells;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nc + 1, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,
                        nc + 1, 0u);
@@ -535,37 +782,73 @@ static RelationArgs relation_args(sphx_relation *rel)
     a.inv_h = rel->inv_h;
     a.inner = rel->inner ? 1 : 0;
     a.neighbor_size = rel->neighbor_size;
-    a.particle_offset = rel->particle_offset;
-    a.neighbor_index = rel->neighbor_index;
+    a.chunk_offset = rel->chunk_offset;
+    a.slice = rel->slice;
+    // pre-filter bounds: float rounding of positions bounded by |x| <= xmax
+    // (target mesh extent + 2.1 h) gives a relative error of the squared
+    // distance below 2.2e-7 xmax/h + 5.6e-7 near the cut-off; the margin is 4x that
+    const MeshParams &m = rel->cll->mesh;
+    double X = 0.0;
+    for (int k = 0; k < rel->source->D; ++k)
+        X = std::max(X, std::max(std::fabs(m.lower[k]), std::fabs(m.lower[k] + m.all_cells[k] * m.gs)));
+    const double h = 1.0 / rel->inv_h;
+    const double xmax = X + 2.1 * h;
+    const double margin = 4.0 * (2.2e-7 * xmax / h + 5.6e-7) + 1e-6;
+    const double c2 = 4.0 / (rel->inv_h * rel->inv_h);
+    a.src_f = rel->source->pos_f32;
+    a.tar_f = rel->target->pos_f32;
+    a.thr_lo = (float)(c2 * (1.0 - margin));
+    a.thr_hi = (float)(c2 * (1.0 + margin));
+    a.pre_xmax = X;
     return a;
 }
 
-sphx_status relation_count(sphx_relation *rel)
+sphx_status relation_to_csr(sphx_relation *rel)
 {
-    sphx_context *ctx = rel->source->ctx;
     uint32_t n = rel->source->n;
     if (n == 0)
         return SPHX_OK;
-    RelationArgs a = relation_args(rel);
-    if (rel->source->D == 2)
-        hipLaunchKernelGGL((k_relation<2, false>), dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL((k_relation<3, false>), dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_slice_to_csr, dim3(grid_for(n, 256)), dim3(256), 0, rel->source->ctx->stream,
+                       rel->neighbor_size, rel->chunk_offset, rel->slice, rel->particle_offset, rel->neighbor_index, n);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
-sphx_status relation_fill(sphx_relation *rel)
+// UpdateRelation in one scan: fixed kmax rows per chunk; *needed > kmax on return
+// asks the caller to grow kmax and run again
+sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed)
 {
-    sphx_context *ctx = rel->source->ctx;
     uint32_t n = rel->source->n;
+    sphx_context *ctx = rel->source->ctx;
     if (n == 0)
         return SPHX_OK;
-    RelationArgs a = relation_args(rel);
+    const uint32_t nchunks = (n + 63u) / 64u;
+    hipLaunchKernelGGL(k_chunk_stride, dim3(grid_for(nchunks + 1, 256)), dim3(256), 0, ctx->stream,
+                       rel->chunk_offset, nchunks, kmax * 64u);
+    BuildArgs b;
+    b.r = relation_args(rel);
+    b.kmax = kmax;
+    b.needed = needed;
+    b.cap = rel->stage_cap;
+    b.stage_count = rel->stage_count;
+    b.stage_keys = rel->stage_keys;
+    b.local_slice = rel->local_slice;
+    ProfileScope prof(ctx, "relation_build");
+    dim3 g(grid_for(n, kThreads)), blk(kThreads);
     if (rel->source->D == 2)
-        hipLaunchKernelGGL((k_relation<2, true>), dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream, a);
+    {
+        if (rel->staged)
+            hipLaunchKernelGGL((k_relation_build<2, true>), g, blk, 0, ctx->stream, b);
+        else
+            hipLaunchKernelGGL((k_relation_build<2, false>), g, blk, 0, ctx->stream, b);
+    }
     else
-        hipLaunchKernelGGL((k_relation<3, true>), dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream, a);
+    {
+        if (rel->staged)
+            hipLaunchKernelGGL((k_relation_build<3, true>), g, blk, 0, ctx->stream, b);
+        else
+            hipLaunchKernelGGL((k_relation_build<3, false>), g, blk, 0, ctx->stream, b);
+    }
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -21,6 +21,7 @@ namespace
 {
 constexpr int kThreads = 256;
 constexpr int kMaxWalls = 4;
+static_assert(kThreads == (int)kStageBlock, "staging blocks are pass workgroups");
 inline unsigned grid_for(size_t n, size_t per) { return (unsigned)((n + per - 1) / per); }
 
 template <class T>
@@ -44,7 +45,9 @@ struct WallArgs
     const T *acc[3];       // wall "Acceleration"
     const T *normal[3];    // wall "NormalDirection"
     const T *Vol_ref;      // wall "VolumetricMeasureRef"
-    const uint32_t *offset, *index;
+    // neighbour lists in 64-wide slices: entry k of particle i at
+    // slice[chunk[i / 64] + 64 k + i % 64], count[i] entries
+    const uint32_t *chunk, *count, *slice;
     const KernelPoly<T> *poly;
 };
 
@@ -64,7 +67,11 @@ struct FluidArgs
     // single-fluid impedance model (riemann_solver_ck.hpp:59-69)
     T rho0c0, inv_rho0c0_sum, inv_rho0c0_ave, rho0c0_geo_ave, inv_c0_ave;
     int riemann;
-    const uint32_t *offset, *index;
+    const uint32_t *chunk, *count, *slice; // inner neighbour slices (see WallArgs)
+    // LDS staging of the inner neighbours (null stage_count: not staged)
+    const uint32_t *stage_count, *stage_keys;
+    const uint16_t *local_slice;
+    uint32_t stage_cap;
     const KernelPoly<T> *poly;
     int n_wall;
     WallArgs<T> wall[kMaxWalls];
@@ -115,6 +122,28 @@ __device__ __forceinline__ T eval_poly(const PolyLDS<T> &lds, T q, T inv_dq, T d
     T t = q - T(loc) * dq;
     const T *c = lds.c[loc];
     return ((c[3] * t + c[2]) * t + c[1]) * t + c[0];
+}
+
+// walk the neighbour entries of one particle (slice layout: entry k at
+// base + 64 k); the indices of a batch are loaded before any of them is used
+// so that their latency overlaps
+template <class I, class F>
+__device__ __forceinline__ void for_each_neighbor(const I *entries, uint32_t base, uint32_t count, F &&f)
+{
+    constexpr uint32_t kBatch = 8;
+    uint32_t k = 0;
+    for (; k + kBatch <= count; k += kBatch)
+    {
+        uint32_t idx[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            idx[q] = entries[base + ((k + q) << 6)];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            f(idx[q]);
+    }
+    for (; k < count; ++k)
+        f((uint32_t)entries[base + (k << 6)]);
 }
 
 //---------------------------------------------------------------------------
@@ -168,11 +197,28 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     using T = typename Prec<P>::T;
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    extern __shared__ __align__(32) unsigned char staged_lds[];
     stage_poly(poly_in, a.poly, true);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, true);
+    const uint32_t blk = xcd_block_index();
+    const uint32_t i = blk * kThreads + threadIdx.x;
+    // staged neighbour records: (x, y, z|Vol, Vol|0) and p of the block's distinct neighbours
+    Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
+    T *s_p = (T *)(s_pv + a.stage_cap);
+    const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
+    const bool staged = n_staged != kStageOverflow;
+    if (staged)
+    {
+        const uint32_t *keys = a.stage_keys + (size_t)blk * a.stage_cap;
+        for (uint32_t r = threadIdx.x; r < n_staged; r += kThreads)
+        {
+            const uint32_t j = keys[r];
+            s_pv[r] = a.posvol[j];
+            s_p[r] = a.p[j];
+        }
+    }
     __syncthreads();
-    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
     if (i >= a.n)
         return;
     const T dt = T(dt_d);
@@ -181,12 +227,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     const T p_i = a.p[i];
     // inner: force_sum -= 2 AverageP dW V_j e_ij, AverageP = (p_i + p_j)/2 for one fluid
     T fx = 0, fy = 0, fz = 0, cd = 0;
-    const uint32_t n0 = a.offset[i], n1 = a.offset[i + 1];
-    for (uint32_t n = n0; n < n1; ++n)
-    {
-        const uint32_t j = a.index[n];
-        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
-        const T p_j = a.p[j];
+    auto pair = [&](const PV<T, D> &pj, const T p_j) {
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
         const T ir = inv_sqrt(r2);
@@ -199,7 +240,12 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
             fz += s * dz;
         if (RIEMANN)
             cd += (p_i - p_j) * dWV;
-    }
+    };
+    const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+    if (staged)
+        for_each_neighbor(a.local_slice, nb, nc, [&](uint32_t r) { pair(load_pv<T, D>(s_pv, r), s_p[r]); });
+    else
+        for_each_neighbor(a.slice, nb, nc, [&](uint32_t j) { pair(load_pv<T, D>(a.posvol, j), a.p[j]); });
     const T k_avg = T(2) * a.inv_rho0c0_sum * a.rho0c0; // 2 * AverageP / (p_i + p_j)
     const T vol_i = pi.vol;
     T Fx = a.force[0][i], Fy = a.force[1][i], Fz = D == 3 ? a.force[2][i] : T(0);
@@ -221,10 +267,10 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
             const WallArgs<T> &w = a.wall[k];
             const T w_inv_h = w.poly->inv_h;
             T wx = 0, wy = 0, wz = 0, wcd = 0;
-            const uint32_t m0 = w.offset[i], m1 = w.offset[i + 1];
-            for (uint32_t n = m0; n < m1; ++n)
+            const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+            for (uint32_t k2 = 0; k2 < mc; ++k2)
             {
-                const uint32_t j = w.index[n];
+                const uint32_t j = w.slice[mb + (k2 << 6)];
                 const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
                 const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
                 const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
@@ -295,11 +341,28 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
     __shared__ double wave_red[kThreads / 64];
+    extern __shared__ __align__(32) unsigned char staged_lds[];
     stage_poly(poly_in, a.poly, true);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, true);
+    const uint32_t blk = xcd_block_index();
+    const uint32_t i = blk * kThreads + threadIdx.x;
+    // staged neighbour records: (x, y, z|Vol, Vol|0) and velocity
+    Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
+    Rec4<T> *s_v = s_pv + a.stage_cap;
+    const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
+    const bool staged = n_staged != kStageOverflow;
+    if (staged)
+    {
+        const uint32_t *keys = a.stage_keys + (size_t)blk * a.stage_cap;
+        for (uint32_t r = threadIdx.x; r < n_staged; r += kThreads)
+        {
+            const uint32_t j = keys[r];
+            s_pv[r] = a.posvol[j];
+            s_v[r] = a.vel[j];
+        }
+    }
     __syncthreads();
-    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
     double signal = -1.0;
     if (i < a.n)
     {
@@ -309,12 +372,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         const Rec4<T> vi = a.vel[i];
         // divergence: 2 (v_i - v_ave).e dW V = (v_i - v_j).e dW V for one fluid
         T div = 0, px = 0, py = 0, pz = 0;
-        const uint32_t n0 = a.offset[i], n1 = a.offset[i + 1];
-        for (uint32_t n = n0; n < n1; ++n)
-        {
-            const uint32_t j = a.index[n];
-            const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
-            const Rec4<T> vj = a.vel[j];
+        auto pair = [&](const PV<T, D> &pj, const Rec4<T> &vj) {
             const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
             const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
             const T ir = inv_sqrt(r2);
@@ -333,7 +391,12 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
                 if (D == 3)
                     pz += s * ez;
             }
-        }
+        };
+        const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+        if (staged)
+            for_each_neighbor(a.local_slice, nb, nc, [&](uint32_t r) { pair(load_pv<T, D>(s_pv, r), s_v[r]); });
+        else
+            for_each_neighbor(a.slice, nb, nc, [&](uint32_t j) { pair(load_pv<T, D>(a.posvol, j), a.vel[j]); });
         const T C_i = full_C<P>(a.C[i]);
         const T vol_i = pi.vol;
         T dC = a.dC[i] + div * C_i;
@@ -343,10 +406,10 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
             const WallArgs<T> &w = a.wall[k];
             const T w_inv_h = w.poly->inv_h;
             T wdiv = 0, wx = 0, wy = 0, wz = 0;
-            const uint32_t m0 = w.offset[i], m1 = w.offset[i + 1];
-            for (uint32_t n = m0; n < m1; ++n)
+            const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+            for (uint32_t k2 = 0; k2 < mc; ++k2)
             {
-                const uint32_t j = w.index[n];
+                const uint32_t j = w.slice[mb + (k2 << 6)];
                 const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
                 const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
                 const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
@@ -439,10 +502,10 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
     const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
     const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
     T s = a.poly->w0 * a.Vol_ref[i];
-    const uint32_t n0 = a.offset[i], n1 = a.offset[i + 1];
-    for (uint32_t n = n0; n < n1; ++n)
+    const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+    for (uint32_t k = 0; k < nc; ++k)
     {
-        const uint32_t j = a.index[n];
+        const uint32_t j = a.slice[nb + (k << 6)];
         const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
@@ -453,9 +516,10 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
     {
         const WallArgs<T> &w = a.wall[k];
         const T w_inv_h = w.poly->inv_h;
-        for (uint32_t n = w.offset[i]; n < w.offset[i + 1]; ++n)
+        const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+        for (uint32_t k2 = 0; k2 < mc; ++k2)
         {
-            const uint32_t j = w.index[n];
+            const uint32_t j = w.slice[mb + (k2 << 6)];
             const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
             const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
             const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
@@ -521,7 +585,7 @@ __global__ void k_setup(FluidArgs<typename Prec<P>::T> a, PosArgs pa, typename P
 
 // UpdateParticlePosition (fluid_time_step_ck.h:176-179) + record refresh
 template <int D, class T>
-__global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol)
+__global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol, Rec4<float> *pos_f32)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pa.n)
@@ -546,10 +610,12 @@ __global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol)
         r.w = T(0);
     }
     posvol[i] = r;
+    if (pos_f32 != (Rec4<float> *)posvol)
+        pos_f32[i] = Rec4<float>{(float)r.x, (float)r.y, D == 3 ? (float)r.z : 0.0f, 0.0f};
 }
 
 template <int D, class T>
-__global__ void k_refresh_posvol(PosArgs pa, const T *Vol, Rec4<T> *posvol)
+__global__ void k_refresh_posvol(PosArgs pa, const T *Vol, Rec4<T> *posvol, Rec4<float> *pos_f32)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pa.n)
@@ -568,6 +634,8 @@ __global__ void k_refresh_posvol(PosArgs pa, const T *Vol, Rec4<T> *posvol)
         r.w = T(0);
     }
     posvol[i] = r;
+    if (pos_f32 != (Rec4<float> *)posvol)
+        pos_f32[i] = Rec4<float>{(float)r.x, (float)r.y, D == 3 ? (float)r.z : 0.0f, 0.0f};
 }
 
 template <int D, class T>
@@ -706,16 +774,18 @@ __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v
 // Interpolation<Contact<Real>>::interact (interpolation_dynamics.hpp:46-60)
 template <int D, class T>
 __global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> *tar, const T *data,
-                              const uint32_t *offset, const uint32_t *index, const KernelPoly<T> *poly, T *out)
+                              const uint32_t *chunk, const uint32_t *count, const uint32_t *slice,
+                              const KernelPoly<T> *poly, T *out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_obs)
         return;
     const PV<T, D> pi = load_pv<T, D>(obs, i);
     T q = 0, w = 0;
-    for (uint32_t n = offset[i]; n < offset[i + 1]; ++n)
+    const uint32_t nb = chunk[i >> 6] + (i & 63u), nc = count[i];
+    for (uint32_t k = 0; k < nc; ++k)
     {
-        const uint32_t j = index[n];
+        const uint32_t j = slice[nb + (k << 6)];
         const PV<T, D> pj = load_pv<T, D>(tar, j);
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
@@ -779,8 +849,16 @@ static void attach_relations(FluidArgs<T> &a, sphx_relation *inner, sphx_relatio
 {
     if (inner)
     {
-        a.offset = inner->particle_offset;
-        a.index = inner->neighbor_index;
+        a.chunk = inner->chunk_offset;
+        a.count = inner->neighbor_size;
+        a.slice = inner->slice;
+        if (inner->staged)
+        {
+            a.stage_count = inner->stage_count;
+            a.stage_keys = inner->stage_keys;
+            a.local_slice = inner->local_slice;
+            a.stage_cap = inner->stage_cap;
+        }
         a.poly = (const KernelPoly<T> *)inner->poly;
     }
     a.n_wall = nw;
@@ -796,8 +874,9 @@ static void attach_relations(FluidArgs<T> &a, sphx_relation *inner, sphx_relatio
             x.normal[d] = (const T *)wb->normal[d];
         }
         x.Vol_ref = (const T *)wb->Vol_ref;
-        x.offset = w[k]->particle_offset;
-        x.index = w[k]->neighbor_index;
+        x.chunk = w[k]->chunk_offset;
+        x.count = w[k]->neighbor_size;
+        x.slice = w[k]->slice;
         x.poly = (const KernelPoly<T> *)w[k]->poly;
     }
 }
@@ -827,6 +906,14 @@ static sphx_status dispatch(sphx_body *b, A &&...args)
     return L<3, P_MIXED>::run(b, args...);
 }
 
+// kernels that take more than 64 KiB of dynamic LDS must say so first
+template <class K>
+static void allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 65536)
+        (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 #define LAUNCH_CHECK()                     \
     do                                     \
     {                                      \
@@ -842,7 +929,7 @@ struct RefreshL
         if (!b->n)
             return SPHX_OK;
         hipLaunchKernelGGL((k_refresh_posvol<D, T>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
-                           pos_args(b), (const T *)b->Vol, (Rec4<T> *)b->posvol);
+                           pos_args(b), (const T *)b->Vol, (Rec4<T> *)b->posvol, b->pos_f32);
         LAUNCH_CHECK();
         return SPHX_OK;
     }
@@ -874,6 +961,7 @@ struct RegularizeL
         using T = typename Prec<P>::T;
         if (!b->n)
             return SPHX_OK;
+        ProfileScope prof(b->ctx, "state_update");
         hipLaunchKernelGGL((k_regularize<D, P>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
                            fluid_args<T>(b), flow);
         LAUNCH_CHECK();
@@ -890,6 +978,7 @@ struct SetupL
         using T = typename Prec<P>::T;
         if (!b->n)
             return SPHX_OK;
+        ProfileScope prof(b->ctx, "state_update");
         hipLaunchKernelGGL((k_setup<D, P>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
                            fluid_args<T>(b), pos_args(b), (T *)b->Vol, (Rec4<T> *)b->posvol);
         LAUNCH_CHECK();
@@ -906,8 +995,9 @@ struct PositionL
         using T = typename Prec<P>::T;
         if (!b->n)
             return SPHX_OK;
+        ProfileScope prof(b->ctx, "state_update");
         hipLaunchKernelGGL((k_update_position<D, T>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
-                           pos_args(b), (const T *)b->Vol, (Rec4<T> *)b->posvol);
+                           pos_args(b), (const T *)b->Vol, (Rec4<T> *)b->posvol, b->pos_f32);
         LAUNCH_CHECK();
         return SPHX_OK;
     }
@@ -924,6 +1014,7 @@ struct SummationL
             return SPHX_OK;
         FluidArgs<T> a = fluid_args<T>(b);
         attach_relations(a, inner, c, nc);
+        ProfileScope prof(b->ctx, "compression_summation");
         hipLaunchKernelGGL((k_summation<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
                            b->ctx->stream, a);
         LAUNCH_CHECK();
@@ -948,16 +1039,33 @@ struct Acoustic1L
         a.riemann = riemann;
         attach_relations(a, inner, w, nw);
         dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
-        hipLaunchKernelGGL((k_init_1st<D, P>), g, blk, 0, b->ctx->stream, a, dt);
+        {
+            ProfileScope prof(b->ctx, "acoustic_init");
+            hipLaunchKernelGGL((k_init_1st<D, P>), g, blk, 0, b->ctx->stream, a, dt);
+        }
+        ProfileScope prof(b->ctx, "acoustic_pass1");
         const bool rs = riemann != SPHX_RIEMANN_NONE;
+        const size_t lds = a.stage_count ? (size_t)a.stage_cap * (sizeof(Rec4<T>) + sizeof(T)) : 0;
         if (rs && fuse_init_2nd)
-            hipLaunchKernelGGL((k_pass1<D, P, true, true>), g, blk, 0, b->ctx->stream, a, dt);
+        {
+            allow_lds(k_pass1<D, P, true, true>, lds);
+            hipLaunchKernelGGL((k_pass1<D, P, true, true>), g, blk, lds, b->ctx->stream, a, dt);
+        }
         else if (rs)
-            hipLaunchKernelGGL((k_pass1<D, P, true, false>), g, blk, 0, b->ctx->stream, a, dt);
+        {
+            allow_lds(k_pass1<D, P, true, false>, lds);
+            hipLaunchKernelGGL((k_pass1<D, P, true, false>), g, blk, lds, b->ctx->stream, a, dt);
+        }
         else if (fuse_init_2nd)
-            hipLaunchKernelGGL((k_pass1<D, P, false, true>), g, blk, 0, b->ctx->stream, a, dt);
+        {
+            allow_lds(k_pass1<D, P, false, true>, lds);
+            hipLaunchKernelGGL((k_pass1<D, P, false, true>), g, blk, lds, b->ctx->stream, a, dt);
+        }
         else
-            hipLaunchKernelGGL((k_pass1<D, P, false, false>), g, blk, 0, b->ctx->stream, a, dt);
+        {
+            allow_lds(k_pass1<D, P, false, false>, lds);
+            hipLaunchKernelGGL((k_pass1<D, P, false, false>), g, blk, lds, b->ctx->stream, a, dt);
+        }
         LAUNCH_CHECK();
         return SPHX_OK;
     }
@@ -977,16 +1085,33 @@ struct Acoustic2L
         attach_relations(a, inner, w, nw);
         dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
         if (run_init)
+        {
+            ProfileScope prof(b->ctx, "acoustic_init");
             hipLaunchKernelGGL((k_init_2nd<D, P>), g, blk, 0, b->ctx->stream, a, dt);
+        }
+        ProfileScope prof(b->ctx, "acoustic_pass2");
         const bool rs = riemann != SPHX_RIEMANN_NONE;
+        const size_t lds = a.stage_count ? (size_t)a.stage_cap * 2 * sizeof(Rec4<T>) : 0;
         if (rs && max_slot)
-            hipLaunchKernelGGL((k_pass2<D, P, true, true>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        {
+            allow_lds(k_pass2<D, P, true, true>, lds);
+            hipLaunchKernelGGL((k_pass2<D, P, true, true>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
+        }
         else if (rs)
-            hipLaunchKernelGGL((k_pass2<D, P, true, false>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        {
+            allow_lds(k_pass2<D, P, true, false>, lds);
+            hipLaunchKernelGGL((k_pass2<D, P, true, false>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
+        }
         else if (max_slot)
-            hipLaunchKernelGGL((k_pass2<D, P, false, true>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        {
+            allow_lds(k_pass2<D, P, false, true>, lds);
+            hipLaunchKernelGGL((k_pass2<D, P, false, true>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
+        }
         else
-            hipLaunchKernelGGL((k_pass2<D, P, false, false>), g, blk, 0, b->ctx->stream, a, dt, max_slot);
+        {
+            allow_lds(k_pass2<D, P, false, false>, lds);
+            hipLaunchKernelGGL((k_pass2<D, P, false, false>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
+        }
         LAUNCH_CHECK();
         return SPHX_OK;
     }
@@ -1021,6 +1146,7 @@ struct SignalL
         using T = typename Prec<P>::T;
         if (!b->n)
             return SPHX_OK;
+        ProfileScope prof(b->ctx, "time_step_reduce");
         hipLaunchKernelGGL((k_signal_max<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
                            b->ctx->stream, fluid_args<T>(b), slot);
         LAUNCH_CHECK();
@@ -1037,6 +1163,7 @@ struct SpeedL
         using T = typename Prec<P>::T;
         if (!b->n)
             return SPHX_OK;
+        ProfileScope prof(b->ctx, "time_step_reduce");
         hipLaunchKernelGGL((k_speed_max<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
                            b->ctx->stream, fluid_args<T>(b), slot);
         LAUNCH_CHECK();
@@ -1089,8 +1216,8 @@ struct InterpL
                         : field == SPHX_FIELD_DENSITY ? (const T *)t->rho
                                                       : (const T *)t->C;
         hipLaunchKernelGGL((k_interpolate<D, T>), dim3(grid_for(o->n, 64)), dim3(64), 0, o->ctx->stream,
-                           (const Rec4<T> *)o->posvol, o->n, (const Rec4<T> *)t->posvol, data, rel->particle_offset,
-                           rel->neighbor_index, (const KernelPoly<T> *)rel->poly, (T *)o->interp);
+                           (const Rec4<T> *)o->posvol, o->n, (const Rec4<T> *)t->posvol, data, rel->chunk_offset,
+                           rel->neighbor_size, rel->slice, (const KernelPoly<T> *)rel->poly, (T *)o->interp);
         LAUNCH_CHECK();
         return SPHX_OK;
     }

@@ -85,6 +85,44 @@ sphx_status transfer_scalar_values(sphx_body *b, void *dev, double *h, bool writ
     return SPHX_OK;
 }
 
+ProfileScope::ProfileScope(sphx_context *c, const char *s) : ctx(c), stage(s)
+{
+    if (ctx->profiling && hipEventCreate(&start) == hipSuccess)
+        (void)hipEventRecord(start, ctx->stream);
+    else
+        start = nullptr;
+}
+ProfileScope::~ProfileScope()
+{
+    if (!start)
+        return;
+    hipEvent_t stop = nullptr;
+    if (hipEventCreate(&stop) != hipSuccess)
+    {
+        (void)hipEventDestroy(start);
+        return;
+    }
+    (void)hipEventRecord(stop, ctx->stream);
+    ctx->pending.push_back({stage, start, stop});
+}
+
+static void drain_profile(sphx_context *ctx)
+{
+    for (auto &e : ctx->pending)
+    {
+        float ms = 0.0f;
+        if (hipEventSynchronize(e.stop) == hipSuccess && hipEventElapsedTime(&ms, e.start, e.stop) == hipSuccess)
+        {
+            auto &t = ctx->profile[e.stage];
+            t.first += ms;
+            t.second += 1;
+        }
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
+    }
+    ctx->pending.clear();
+}
+
 namespace
 {
 template <class T>
@@ -307,6 +345,9 @@ sphx_status sphx_context_create(int device, sphx_context **out)
     SPHX_HIP(hipSetDevice(device));
     std::unique_ptr<sphx_context> ctx(new sphx_context());
     ctx->device = device;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        ctx->max_lds = (size_t)lds;
     SPHX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     SPHX_HIP(hipMalloc(&ctx->scalars_d, kScalarSlots * sizeof(unsigned long long)));
     SPHX_HIP(hipHostMalloc(&ctx->scalars_h, kScalarSlots * sizeof(unsigned long long), hipHostMallocDefault));
@@ -323,6 +364,7 @@ sphx_status sphx_context_destroy(sphx_context *ctx)
         return SPHX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    drain_profile(ctx);
     (void)hipFree(ctx->scalars_d);
     (void)hipHostFree(ctx->scalars_h);
     (void)hipStreamDestroy(ctx->stream);
@@ -335,6 +377,33 @@ sphx_status sphx_context_synchronize(sphx_context *ctx)
     if (!ctx)
         return arg_error("null context");
     SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    return SPHX_OK;
+}
+
+sphx_status sphx_profile_enable(sphx_context *ctx, int enable)
+{
+    if (!ctx)
+        return arg_error("null context");
+    SPHX_HIP(hipSetDevice(ctx->device));
+    SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    drain_profile(ctx);
+    ctx->profile.clear();
+    ctx->profiling = enable != 0;
+    return SPHX_OK;
+}
+
+sphx_status sphx_profile_read(sphx_context *ctx, const char *stage, double *total_ms, uint64_t *launches)
+{
+    if (!ctx || !stage)
+        return arg_error("null argument");
+    SPHX_HIP(hipSetDevice(ctx->device));
+    SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    drain_profile(ctx);
+    auto it = ctx->profile.find(stage);
+    if (total_ms)
+        *total_ms = it == ctx->profile.end() ? 0.0 : it->second.first;
+    if (launches)
+        *launches = it == ctx->profile.end() ? 0 : it->second.second;
     return SPHX_OK;
 }
 
@@ -377,6 +446,11 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
     for (void **f : scalars)
         if ((s = alloc(b.get(), f, n * T)))
             return s;
+    if (b->P == P_FP64)
+    {
+        if ((s = alloc(b.get(), (void **)&b->pos_f32, n * sizeof(Rec4<float>))))
+            return s;
+    }
     if ((s = alloc(b.get(), &b->posvol, n * R4)) || (s = alloc(b.get(), &b->vel, n * R4)) ||
         (s = alloc(b.get(), (void **)&b->original_id, n * 4)) || (s = alloc(b.get(), (void **)&b->sorted_id, n * 4)) ||
         (s = alloc(b.get(), (void **)&b->sequence, n * 4)) || (s = alloc(b.get(), (void **)&b->perm, n * 4)))
@@ -392,6 +466,10 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
     for (void **f : scalars)
         SPHX_HIP(hipMemsetAsync(*f, 0, n * T, ctx->stream));
     SPHX_HIP(hipMemsetAsync(b->posvol, 0, n * R4, ctx->stream));
+    if (b->P == P_MIXED)
+        b->pos_f32 = (Rec4<float> *)b->posvol; // (x, y, z|Vol) in float already
+    else
+        SPHX_HIP(hipMemsetAsync(b->pos_f32, 0, n * sizeof(Rec4<float>), ctx->stream));
     SPHX_HIP(hipMemsetAsync(b->vel, 0, n * R4, ctx->stream));
     SPHX_HIP(hipMemsetAsync(b->sequence, 0, n * 4, ctx->stream));
     if (b->P == P_FP64)
@@ -611,7 +689,24 @@ static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_lin
     SPHX_HIP(hipSetDevice(src->ctx->device));
     SPHX_HIP(hipMalloc(&r->neighbor_size, (size_t)(src->n + 1) * sizeof(uint32_t)));
     SPHX_HIP(hipMalloc(&r->particle_offset, (size_t)(src->n + 1) * sizeof(uint32_t)));
+    SPHX_HIP(hipMalloc(&r->chunk_offset, (size_t)((src->n + 63) / 64 + 1) * sizeof(uint32_t)));
     SPHX_HIP(hipMemsetAsync(r->neighbor_size, 0, (size_t)(src->n + 1) * sizeof(uint32_t), src->ctx->stream));
+    if (inner && src->desc.kind == SPHX_BODY_FLUID && src->n > 0)
+    {
+        // LDS staging of the acoustic passes: the 2nd pass stages two 4-wide
+        // records per distinct neighbour; keep 8 KiB for the static tables
+        // (MIXED only: FP64 records would halve the occupancy of the passes)
+        const size_t rec2 = 2 * (src->P == P_FP64 ? sizeof(Rec4<double>) : sizeof(Rec4<float>));
+        const size_t cap = 1536;
+        if (src->P == P_MIXED && cap * rec2 + 8192 <= std::min<size_t>(src->ctx->max_lds, 65536))
+        {
+            const size_t nblocks = (src->n + kStageBlock - 1) / kStageBlock;
+            r->staged = true;
+            r->stage_cap = (uint32_t)cap;
+            SPHX_HIP(hipMalloc(&r->stage_count, nblocks * sizeof(uint32_t)));
+            SPHX_HIP(hipMalloc(&r->stage_keys, nblocks * cap * sizeof(uint32_t)));
+        }
+    }
     sphx_status s = build_kernel_poly(r.get());
     if (s)
         return s;
@@ -645,6 +740,15 @@ sphx_status sphx_relation_destroy(sphx_relation *r)
     (void)hipFree(r->particle_offset);
     if (r->neighbor_index)
         (void)hipFree(r->neighbor_index);
+    (void)hipFree(r->chunk_offset);
+    if (r->slice)
+        (void)hipFree(r->slice);
+    if (r->stage_count)
+        (void)hipFree(r->stage_count);
+    if (r->stage_keys)
+        (void)hipFree(r->stage_keys);
+    if (r->local_slice)
+        (void)hipFree(r->local_slice);
     if (r->poly)
         (void)hipFree(r->poly);
     delete r;
@@ -661,26 +765,54 @@ sphx_status sphx_update_relation(sphx_relation *r)
     sphx_context *ctx = r->source->ctx;
     SPHX_HIP(hipSetDevice(ctx->device));
     sphx_status s;
-    // count -> exclusive scan over n+1 (last count is 0) -> grow -> fill
-    if ((s = relation_count(r)))
-        return s;
-    uint32_t total = 0;
-    if ((s = exclusive_scan_u32(ctx, r->neighbor_size, r->particle_offset, (size_t)r->source->n + 1, &total)))
-        return s;
-    if (total > r->capacity || !r->neighbor_index)
+    // one scan builds the slices (and the staging sets); grow the slice width
+    // and rebuild when a particle has more neighbours than it, as the
+    // reference grows NeighborIndex on overflow (update_body_relation.hpp:145-155)
+    const size_t n = r->source->n;
+    const size_t nchunks = (n + 63) / 64;
+    uint32_t kmax = r->kmax ? r->kmax : (r->source->D == 3 ? 96u : 32u);
+    for (int attempt = 0; attempt < 4; ++attempt)
     {
-        // reference grows NeighborIndex on overflow (update_body_relation.hpp:145-155)
-        SPHX_HIP(hipStreamSynchronize(ctx->stream));
-        if (r->neighbor_index)
-            SPHX_HIP(hipFree(r->neighbor_index));
-        r->neighbor_index = nullptr;
-        size_t cap = std::max<size_t>((size_t)total + total / 4, 64);
-        SPHX_HIP(hipMalloc(&r->neighbor_index, cap * sizeof(uint32_t)));
-        r->capacity = cap;
+        const size_t need = std::max<size_t>(nchunks * kmax * 64, 64);
+        if (need >= (1ull << 32))
+        {
+            set_error("neighbour slices exceed 2^32 entries");
+            return SPHX_E_CAPACITY;
+        }
+        if (need > r->slice_capacity || !r->slice)
+        {
+            SPHX_HIP(hipStreamSynchronize(ctx->stream));
+            if (r->slice)
+                SPHX_HIP(hipFree(r->slice));
+            r->slice = nullptr;
+            if (r->local_slice)
+                SPHX_HIP(hipFree(r->local_slice));
+            r->local_slice = nullptr;
+            SPHX_HIP(hipMalloc(&r->slice, need * sizeof(uint32_t)));
+            if (r->staged)
+                SPHX_HIP(hipMalloc(&r->local_slice, need * sizeof(uint16_t)));
+            r->slice_capacity = need;
+        }
+        if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_NEEDED, 1, 0ull)) ||
+            (s = relation_build(r, kmax, ctx->scalars_d + SLOT_NEEDED)) ||
+            (s = read_scalars(ctx, SLOT_NEEDED, 1)))
+            return s;
+        const unsigned long long needed = ctx->scalars_h[SLOT_NEEDED];
+        if (needed <= kmax)
+            break;
+        if (needed > 60000)
+        {
+            set_error("a particle has more than 60000 neighbours: positions collapsed");
+            return SPHX_E_CAPACITY;
+        }
+        kmax = (uint32_t)((needed + needed / 4 + 7) / 8 * 8);
     }
-    r->total = total;
-    if ((s = relation_fill(r)))
+    r->kmax = kmax;
+    r->slice_total = nchunks * kmax * 64;
+    // CSR offsets of the reference layout (NeighborSize scan), for reads
+    if ((s = exclusive_scan_u32(ctx, r->neighbor_size, r->particle_offset, n + 1, nullptr)))
         return s;
+    r->csr_valid = false;
     r->built = true;
     r->built_version = r->source->config_version;
     r->target_version = r->target->config_version;
@@ -694,8 +826,29 @@ sphx_status sphx_relation_read(sphx_relation *r, uint32_t *size, uint32_t *offse
         return arg_error("null relation");
     if (!relation_ready(r))
         return state_error("relation not updated since the last configuration change");
-    hipStream_t s = r->source->ctx->stream;
+    sphx_context *ctx = r->source->ctx;
+    hipStream_t s = ctx->stream;
+    SPHX_HIP(hipSetDevice(ctx->device));
     size_t n = r->source->n;
+    if (!r->csr_valid)
+    {
+        uint32_t t = 0;
+        SPHX_HIP(hipMemcpyAsync(&t, r->particle_offset + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SPHX_HIP(hipStreamSynchronize(s));
+        r->total = t;
+        if (t > r->capacity || !r->neighbor_index)
+        {
+            if (r->neighbor_index)
+                SPHX_HIP(hipFree(r->neighbor_index));
+            r->neighbor_index = nullptr;
+            r->capacity = std::max<size_t>(t, 64);
+            SPHX_HIP(hipMalloc(&r->neighbor_index, r->capacity * sizeof(uint32_t)));
+        }
+        sphx_status st = relation_to_csr(r);
+        if (st)
+            return st;
+        r->csr_valid = true;
+    }
     if (total)
         *total = r->total;
     if (size && n)
@@ -721,6 +874,7 @@ sphx_status sphx_particle_sort(sphx_body *b, sphx_cell_linked_list *c)
     if (!b || !c || c->body != b)
         return arg_error("sort needs the body's own cell linked list");
     SPHX_HIP(hipSetDevice(b->ctx->device));
+    ProfileScope prof(b->ctx, "particle_sort");
     sphx_status s;
     if ((s = morton_sequence(b, c)))
         return s;

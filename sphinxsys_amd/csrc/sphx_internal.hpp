@@ -4,6 +4,7 @@
 #include "../../include/sphx.h"
 #include "sphx_device.hpp"
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -30,12 +31,16 @@ struct Scratch
 };
 
 constexpr int kScalarSlots = 64;
+constexpr uint32_t kStageBlock = 256;          // source particles per staging block
+constexpr uint32_t kStageOverflow = 0xffffffffu;
+constexpr uint32_t kStageMaxCap = 2048;        // largest staged set the localize kernel sorts
 enum ScalarSlot
 {
     SLOT_MAX_A = 0,      // ordered-bits max accumulators (u64)
     SLOT_MAX_B = 1,
     SLOT_TOTAL = 2,      // u32 totals returned by scans (stored as u64)
     SLOT_DT = 3,         // double: acoustic dt of the current sub-step
+    SLOT_NEEDED = 4,     // u64: neighbour count beyond the slice width
     SLOT_SUM_BASE = 8    // double partial sums region start (reduce_sum final)
 };
 } // namespace sphx
@@ -44,6 +49,16 @@ struct sphx_context
 {
     int device = 0;
     hipStream_t stream = nullptr;
+    size_t max_lds = 65536; // shared memory per workgroup of the device
+    // HIP-event kernel timing (sphx_profile_*)
+    bool profiling = false;
+    struct PendingEvent
+    {
+        std::string stage;
+        hipEvent_t start, stop;
+    };
+    std::vector<PendingEvent> pending;
+    std::map<std::string, std::pair<double, uint64_t>> profile; // stage -> (ms, launches)
     unsigned long long *scalars_d = nullptr; // kScalarSlots u64 on the device
     unsigned long long *scalars_h = nullptr; // pinned mirror
     sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp;
@@ -63,6 +78,7 @@ struct sphx_body
     double *dpos[3] = {nullptr, nullptr, nullptr};
     void *posvol = nullptr; // Rec4<T>
     void *vel = nullptr;    // Rec4<T>
+    sphx::Rec4<float> *pos_f32 = nullptr; // float (x, y, z|., .) for the neighbour pre-filter (aliases posvol in MIXED)
     void *force[3] = {nullptr, nullptr, nullptr};
     void *force_prior[3] = {nullptr, nullptr, nullptr};
     void *normal[3] = {nullptr, nullptr, nullptr};
@@ -92,10 +108,27 @@ struct sphx_relation
     bool inner = true;
     sphx_body *source = nullptr, *target = nullptr;
     sphx_cell_linked_list *cll = nullptr; // of the target
-    uint32_t *neighbor_size = nullptr;    // n_src
-    uint32_t *particle_offset = nullptr;  // n_src + 1
-    uint32_t *neighbor_index = nullptr;   // capacity
+    uint32_t *neighbor_size = nullptr;    // n_src + 1 (last entry stays 0)
+    uint32_t *particle_offset = nullptr;  // n_src + 1, CSR offsets of the reference layout
+    uint32_t *neighbor_index = nullptr;   // CSR NeighborIndex, built on read only
     size_t capacity = 0, total = 0;
+    // device layout used by the kernels: 64-wide slices, entry k of particle i
+    // at slice[chunk_offset[i / 64] + 64 k + i % 64]
+    uint32_t *chunk_offset = nullptr;     // n_chunks + 1
+    uint32_t *slice = nullptr;
+    size_t slice_capacity = 0, slice_total = 0;
+    uint32_t kmax = 0;                    // rows per 64-particle chunk of the slices
+    bool csr_valid = false;
+    // LDS staging of the inner relation: per block of kStageBlock source
+    // particles, the sorted set of distinct neighbours (stage_keys[b * stage_cap + r],
+    // stage_count[b] of them, or kStageOverflow) and the neighbour entries as
+    // 16-bit ranks into that set (local_slice, same slice layout)
+    bool staged = false;
+    uint32_t stage_cap = 0;
+    uint32_t *stage_count = nullptr;
+    uint32_t *stage_keys = nullptr;
+    uint16_t *local_slice = nullptr;
+    size_t local_capacity = 0;
     bool built = false;
     uint64_t built_version = 0, target_version = 0; // config versions the lists were built for
     double inv_h = 0, src_inv_h = 0;      // Neighbor<SPHAdaptation,SPHAdaptation> constants
@@ -109,8 +142,8 @@ sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *
                                uint32_t *total_host);
 sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
 sphx_status cell_linked_list_update(sphx_cell_linked_list *cll);
-sphx_status relation_count(sphx_relation *rel);
-sphx_status relation_fill(sphx_relation *rel);
+sphx_status relation_to_csr(sphx_relation *rel);
+sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed);
 sphx_status morton_sequence(sphx_body *b, sphx_cell_linked_list *cll);
 sphx_status gather_u32(sphx_context *ctx, uint32_t *field, const uint32_t *perm, size_t n);
 sphx_status gather_real(sphx_body *b, void **field, const uint32_t *perm);
@@ -144,4 +177,15 @@ sphx_status build_kernel_poly(sphx_relation *rel);
 
 // ---- scalar read-back helper (sphx_api.hip) ----
 sphx_status read_scalars(sphx_context *ctx, int first, int count);
+
+// records HIP events around the launches issued in its scope when the
+// context is profiling
+struct ProfileScope
+{
+    sphx_context *ctx;
+    const char *stage;
+    hipEvent_t start = nullptr;
+    ProfileScope(sphx_context *c, const char *s);
+    ~ProfileScope();
+};
 } // namespace sphx

@@ -133,3 +133,17 @@ def normal_from_body_shape(shape, pos):
         raise ValueError("normal requested on the shape surface")
     direction = disp / np.sqrt(z)[:, None]
     return np.where(is_contain[:, None], direction, -1.0 * direction)
+
+
+def morton_order(pos, bounds_lower, grid_spacing, buffer_width=2):
+    """Stable permutation that orders particles by the Morton code of their
+    cell on the cell-linked-list mesh (the ParticleSortCK sequence,
+    meshes/base_mesh.hxx:85-99, cell_linked_list.cpp:131-136).  Used to emit
+    generated lattices in a cache-friendly order."""
+    lower = np.asarray(bounds_lower, dtype=np.float64) - buffer_width * grid_spacing
+    c = np.maximum(np.floor((pos - lower) / grid_spacing).astype(np.int64), 0)
+    key = np.zeros(len(pos), dtype=np.uint64)
+    for bit in range(10):
+        for axis in range(pos.shape[1]):
+            key |= ((c[:, axis].astype(np.uint64) >> bit) & 1) << (bit * 3 + axis)
+    return np.argsort(key, kind="stable")

@@ -5,7 +5,7 @@ from sphinxsys_amd import cases, geometry
 
 
 def test_dambreak_2d_particle_counts():
-    g = cases.dambreak_2d()
+    g = cases.dambreak_2d(order="lattice")
     assert len(g.fluid_pos) == 80 * 40
     assert len(g.wall_pos) == 3504
     assert np.allclose(g.fluid_pos.min(axis=0), [0.0125, 0.0125])
@@ -37,3 +37,11 @@ def test_dambreak_3d_small_counts():
     n = len(g.fluid_pos)
     assert n == 24 * 20 * 12
     assert g.wall_normal.shape == g.wall_pos.shape
+
+
+def test_morton_order_is_a_permutation_sorted_by_cell_key():
+    g = cases.dambreak_2d(order="lattice")
+    m = cases.dambreak_2d(order="morton")
+    assert np.array_equal(np.sort(g.fluid_pos, axis=0), np.sort(m.fluid_pos, axis=0))
+    perm = geometry.morton_order(g.fluid_pos, g.bounds_lower, 2.6 * g.dp)
+    assert np.array_equal(m.fluid_pos, g.fluid_pos[perm])
