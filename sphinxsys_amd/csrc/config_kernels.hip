@@ -369,7 +369,7 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t *ht, uint32_t key)
 // Rows of a wave are assembled in an LDS ring buffer and written out whole, so
 // the list writes are coalesced rows instead of scattered 4-byte stores.
 //---------------------------------------------------------------------------
-constexpr uint32_t kRing = 32;      // ring rows per wave
+constexpr uint32_t kRing = 16;      // ring rows per wave
 constexpr uint32_t kFlushEvery = 8; // candidate iterations between flushes
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
