@@ -453,3 +453,159 @@ def test_time_steps_and_energy_closed_form(oracle, dim):
     assert ck.TotalMechanicalEnergyCK(b, g).exec() == pytest.approx(e, rel=1e-12)
     assert ck.UpperFrontInAxisDirectionCK(b, 0).exec() == pos[:, 0].max()
     ctx.close()
+
+
+# ---------------------------------------------------------------- streaming updates
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("flow", [capi.FLOW_FREE_SURFACE, capi.FLOW_INTERNAL])
+def test_summation_regularization_setup_position_closed_form(oracle, dim, flow):
+    """Isolated particles: S = W0 Vref (W0 = dimfac / h^d, as Wendland(0) = 1);
+    a dense interior lattice: S > 1.  FreeSurface: C = max(S, 1); Internal:
+    C = S; rho = C rho0 (density_regularization.hpp:85-118).  Setup: V = m/rho,
+    dpos = 0; position update: x += dpos (fluid_time_step_ck.h:133-184).
+    Gravity: F_prior = m g (force_prior_ck.hpp:37-43)."""
+    dp, rho0 = 0.1, 1000.0
+    h = 1.3 * dp
+    m = 7 if dim == 2 else 5
+    axes = [np.arange(m) * dp for _ in range(dim)]
+    lattice = np.stack([g.ravel() for g in np.meshgrid(*axes, indexing="ij")], axis=1)
+    lone = np.full((2, dim), 10.0)
+    lone[1, 0] += 1.0
+    pos = np.vstack([lattice, lone])
+    n = len(pos)
+    ctx, b = make_body(oracle, pos, dp, rho0=rho0)
+    lo, up = domain(pos)
+    cll = ck.CellLinkedList(b, lo, up)
+    inner = ck.Inner(b, cll)
+    ck.UpdateCellLinkedList(cll).exec()
+    ck.UpdateRelation(inner).exec()
+    mass = np.linspace(0.9, 1.1, n)
+    b.write("Mass", mass)
+    g = np.zeros(dim)
+    g[-1] = -9.81
+    ck.GravityForceCK(b, g).exec()
+    assert np.allclose(b.read("ForcePrior"), mass[:, None] * g[None, :], rtol=1e-15)
+    ck.CompressionSummation(inner).exec()
+    S = b.read("CompressionSummation")
+    V = dp ** dim
+    assert S[-1] == pytest.approx(dimfac(dim) / h ** dim * V, rel=1e-14)
+    assert S[-2] == pytest.approx(S[-1], rel=1e-15)
+    centre = np.argmin(np.sum((lattice - lattice.mean(axis=0)) ** 2, axis=1))
+    assert S[centre] > 1.0 > S[-1]
+    ck.DensityRegularization(b, flow).exec()
+    C = b.read("Compression")
+    expect = np.maximum(S, 1.0) if flow == capi.FLOW_FREE_SURFACE else S
+    assert np.array_equal(C, expect)
+    assert np.allclose(b.read("Density"), expect * rho0, rtol=1e-15)
+    b.write("Displacement", np.ones((n, dim)))
+    ck.AdvectionStepSetup(b).exec()
+    assert np.allclose(b.read("VolumetricMeasure"), mass / (expect * rho0), rtol=1e-15)
+    assert not b.read("Displacement").any()
+    rng = np.random.default_rng(3)
+    dpos = rng.normal(scale=1e-3, size=(n, dim))
+    b.write("Displacement", dpos)
+    ck.UpdateParticlePosition(b).exec()
+    assert np.array_equal(b.read("Position"), pos + dpos)
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_riemann_variants_two_particle(oracle, dim):
+    """NoRiemannSolverCK: no dissipation (F of the 2nd half = 0, no pressure
+    jump in the 1st-half continuity); the plain dissipative solver has no
+    limiter: PJump = rho0 c0 u (riemann_solver_ck.h:46-173)."""
+    rho0, c0 = 1.0, 10.0
+    vel = np.zeros((2, dim))
+    vel[0, 0] = -0.5  # u = 0.5 + 0.2 > 0 with 3u/c0 = 0.21 < 1: the limiter matters
+    vel[1, 0] = 0.2
+    e01 = np.zeros(dim)
+    e01[0] = -1.0
+    u = np.dot(vel[0] - vel[1], e01)
+    out = {}
+    for kind in (capi.RIEMANN_ACOUSTIC, capi.RIEMANN_NONE, capi.RIEMANN_DISSIPATIVE):
+        ctx, b, inner, dW, V = two_particle_scene(oracle, dim, 11, rho0, c0)
+        b.write("Compression", np.array([1.01, 1.0]))
+        ck.AcousticStep1stHalf(inner, riemann=kind).exec(0.0)
+        dC1 = b.read("CompressionRate")[0]
+        b.write("Velocity", vel)
+        b.write("CompressionRate", np.zeros(2))
+        ck.AcousticStep2ndHalf(inner, riemann=kind).exec(0.0)
+        out[kind] = (dC1, b.read("Force")[0], dW, V)
+        ctx.close()
+    dC1, F, dW, V = out[capi.RIEMANN_NONE]
+    assert dC1 == 0.0 and not F.any()
+    p0 = rho0 * c0 ** 2 * 0.01
+    for kind in (capi.RIEMANN_ACOUSTIC, capi.RIEMANN_DISSIPATIVE):
+        assert out[kind][0] == pytest.approx(1.01 * p0 / (rho0 * c0) * dW * V, rel=1e-10)
+    limiter = min(3.0 * max(u, 0.0) / c0, 1.0)
+    assert np.allclose(out[capi.RIEMANN_ACOUSTIC][1], V * rho0 * c0 * u * limiter * dW * V * e01, rtol=1e-10)
+    assert np.allclose(out[capi.RIEMANN_DISSIPATIVE][1], V * rho0 * c0 * u * dW * V * e01, rtol=1e-10)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_observer_interpolation_closed_form(oracle, dim):
+    """Interpolation<Contact<Real>>: p_obs = sum W V p / (sum W V + TinyReal)
+    (interpolation_dynamics.hpp:46-60), with the targets at table nodes so
+    that the tabulated W equals the closed form."""
+    dp = 0.1
+    h = 1.3 * dp
+    nodes = [3, 7, 12, 16]
+    tpos = np.zeros((len(nodes), dim))
+    for k, q in enumerate(nodes):
+        tpos[k, k % dim] = (1 if k < 2 else -1) * q * 0.1 * h
+    ctx = ck.Context(oracle)
+    tb = ck.Body(ctx, dim, len(nodes), dp)
+    tb.write("Position", tpos)
+    vol = np.array([1.0, 0.8, 1.2, 0.9]) * dp ** dim
+    tb.write("VolumetricMeasure", vol)
+    C = np.array([1.001, 1.002, 0.999, 1.0005])
+    tb.write("Compression", C)
+    ob = ck.Body(ctx, dim, 1, dp, capi.BODY_OBSERVER)
+    ob.write("Position", np.zeros((1, dim)))
+    lo, up = domain(np.vstack([tpos, np.zeros((1, dim))]))
+    tc = ck.CellLinkedList(tb, lo, up)
+    rel = ck.Contact(ob, tb, tc)
+    ck.UpdateCellLinkedList(tc).exec()
+    ck.UpdateRelation(rel).exec()
+    p = 1.0 * 1.0 ** 2 * (C - 1.0)  # rho0 c0^2 (C - 1)
+    tb.write("Pressure", p)
+    w = dimfac(dim) / h ** dim * np.array([wendland_w(q * 0.1) for q in nodes]) * vol
+    got = ck.InterpolationContact(rel, "Pressure").exec()[0]
+    assert got == pytest.approx(np.sum(w * p) / (np.sum(w) + np.finfo(np.float64).eps), rel=1e-12)
+    ctx.close()
+
+
+def test_fp32_restatement_against_fp64(oracle):
+    """The FP32 restatement (the reference's SPHINXSYS_USE_FLOAT build) on the
+    same 3D scene as the pinned FP64 one: neighbour lists identical away from
+    the cut-off, summation and pair forces within float rounding.  The
+    compression rate sums p_i - p_j of neighbours whose C differs by ~1e-5, so
+    float storage of C (6e-8 absolute) costs it ~1e-4 relative: gate 1e-3."""
+    rng = np.random.default_rng(12)
+    dp = 0.05
+    m = 9
+    axes = [np.arange(m) * dp for _ in range(3)]
+    pos = np.stack([g.ravel() for g in np.meshgrid(*axes, indexing="ij")], axis=1)
+    pos += rng.uniform(-0.1 * dp, 0.1 * dp, size=pos.shape)
+    res = {}
+    for prec in (capi.FP64, 2):
+        ctx, b = make_body(oracle, pos, dp, rho0=1000.0, c0=20.0, precision=prec)
+        lo, up = domain(pos)
+        cll = ck.CellLinkedList(b, lo, up)
+        inner = ck.Inner(b, cll)
+        ck.UpdateCellLinkedList(cll).exec()
+        ck.UpdateRelation(inner).exec()
+        b.write("Compression", 1.0 + np.linspace(-0.005, 0.005, len(pos)))
+        b.write("Velocity", np.outer(np.linspace(-1, 1, len(pos)), np.ones(3)) * 0.1)
+        b.write("Mass", np.full(len(pos), 1000.0 * dp ** 3))
+        ck.CompressionSummation(inner).exec()
+        ck.AcousticStep1stHalf(inner).exec(1e-5)
+        res[prec] = (inner.read(), b.read("CompressionSummation"), b.read("Force"), b.read("CompressionRate"))
+        ctx.close()
+    (r64, s64, f64, c64), (r32, s32, f32, c32) = res[capi.FP64], res[2]
+    for a, b_ in zip(r64, r32):
+        assert np.array_equal(a, b_)
+    rel = lambda a, b_: np.linalg.norm(a - b_) / np.linalg.norm(b_)
+    assert rel(s32, s64) < 1e-6
+    assert rel(f32, f64) < 1e-4
+    assert rel(c32, c64) < 1e-3
