@@ -72,7 +72,7 @@ def run_gpu(lib, geom, precision, warmup, steps):
     n_ac = sc.acoustic_steps - a0
     prof = {k: sc.ctx.profile_read(k) for k in (
         "acoustic_pass1", "acoustic_pass2", "acoustic_init", "compression_summation",
-        "relation_build", "cell_linked_list", "particle_sort",
+        "relation_build", "relation_scan", "relation_stage", "cell_linked_list", "particle_sort",
         "time_step_reduce", "state_update")}
     sc.ctx.profile(False)
     inner_size, _, _ = sc.inner.read()

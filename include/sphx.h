@@ -246,7 +246,9 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
  * launch of the named stage (no reference counterpart; the reference times
  * whole phases with TickCount, dambreak_sycl.cpp:175-259).  Stage names:
  * "acoustic_pass1", "acoustic_pass2", "acoustic_init", "compression_summation",
- * "relation_build", "cell_linked_list", "particle_sort",
+ * "relation_build" (tiled list build), "relation_scan" (per-lane scan build,
+ * used when a list outgrows the LDS tile), "relation_stage" (LDS staging sets
+ * of the FP32-mixed inner relation), "cell_linked_list", "particle_sort",
  * "time_step_reduce", "state_update".  Reading synchronises the stream and
  * returns the accumulated milliseconds and launch count since enabling. */
 sphx_status sphx_profile_enable(sphx_context *ctx, int enable);

@@ -362,6 +362,57 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t *ht, uint32_t key)
     return s;
 }
 
+// Collects the occupied slots of the block's hash set into keys[0, total) and
+// sorts them ascending (bitonic, padded to a power of two with kEmpty).
+// Returns false, with keys untouched, when the set overflowed the hash or has
+// more than `cap` entries.  Called by all threads of a 256-thread block.
+__device__ bool block_collect_sorted(const uint32_t *ht, uint32_t *keys, uint32_t cap, uint32_t hash_full,
+                                     uint32_t *total_out)
+{
+    const uint32_t t = threadIdx.x;
+    constexpr uint32_t per = kHashSize / kThreads;
+    uint32_t occ = 0;
+    for (uint32_t q = 0; q < per; ++q)
+        occ += ht[t * per + q] != kEmpty;
+    uint32_t total;
+    uint32_t pos = block_exclusive_scan(occ, &total);
+    *total_out = total;
+    if (total > cap || total > kStageMaxCap || hash_full)
+        return false;
+    for (uint32_t q = 0; q < per; ++q)
+    {
+        const uint32_t v = ht[t * per + q];
+        if (v != kEmpty)
+            keys[pos++] = v;
+    }
+    uint32_t pow2 = 1;
+    while (pow2 < total)
+        pow2 <<= 1;
+    for (uint32_t q = total + t; q < pow2; q += kThreads)
+        keys[q] = kEmpty;
+    __syncthreads();
+    for (uint32_t size = 2; size <= pow2; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+        {
+            for (uint32_t q = t; q < pow2; q += kThreads)
+            {
+                const uint32_t partner = q ^ stride;
+                if (partner > q)
+                {
+                    const bool up = (q & size) == 0;
+                    const uint32_t x = keys[q], y = keys[partner];
+                    if ((x > y) == up)
+                    {
+                        keys[q] = y;
+                        keys[partner] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    return true;
+}
+
 //---------------------------------------------------------------------------
 // one-pass relation build: a single candidate scan per source particle writes
 // the neighbour slices (fixed kmax rows per 64-particle chunk) and, for staged
@@ -392,35 +443,17 @@ struct BuildArgs
     RelationArgs r;
     uint32_t kmax;
     unsigned long long *needed; // max neighbour count seen when it exceeds kmax
-    uint32_t cap;
-    uint32_t *stage_count, *stage_keys;
-    uint16_t *local_slice;
 };
 
-template <int D, bool STAGE>
+template <int D>
 __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
 {
     const RelationArgs &a = b.r;
-    // ring rows during the scan; reused for the sorted keys and the rank table after it
-    __shared__ uint32_t scratch[(kThreads / 64) * kRing * 64];
-    __shared__ uint32_t ht[STAGE ? kHashSize : 1];
-    __shared__ uint32_t hash_full;
-    static_assert(kStageMaxCap + kHashSize / 2 <= (kThreads / 64) * kRing * 64, "keys + rank fit in the ring");
-    uint32_t (*ring)[kRing][64] = reinterpret_cast<uint32_t (*)[kRing][64]>(scratch);
-    uint32_t *keys = scratch;                                          // after the scan
-    uint16_t *rank = reinterpret_cast<uint16_t *>(scratch + kStageMaxCap); // after the scan
+    __shared__ uint32_t ring[kThreads / 64][kRing][64];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + t;
     const bool valid = i < a.n_src;
-    if (STAGE)
-    {
-        for (uint32_t k = t; k < kHashSize; k += kThreads)
-            ht[k] = kEmpty;
-        if (t == 0)
-            hash_full = 0;
-        __syncthreads();
-    }
     double xi[3] = {0.0, 0.0, 0.0};
     int c[3] = {0, 0, 0};
     Rec4<float> xf = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -449,10 +482,10 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
         flushed = max(flushed, target);
     };
     // one candidate: float pre-filter, exact double criterion in the margin
-    // shell, append to the ring; returns the neighbour or kEmpty
-    auto test = [&](uint32_t j, const Rec4<float> &pf) -> uint32_t {
+    // shell, append to the ring
+    auto test = [&](uint32_t j, const Rec4<float> &pf) {
         if (a.inner && j == i)
-            return kEmpty;
+            return;
         const float fx = xf.x - pf.x, fy = xf.y - pf.y, fz = D == 3 ? xf.z - pf.z : 0.0f;
         float r2f = fx * fx + fy * fy;
         if (D == 3)
@@ -473,7 +506,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
             }
         }
         if (!(r2 < 4.0))
-            return kEmpty;
+            return;
         if (k < b.kmax)
         {
             if (k < flushed)
@@ -482,28 +515,6 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
                 ring[wave][k % kRing][lane] = j;
         }
         ++k;
-        return j;
-    };
-    auto insert = [&](uint32_t found) {
-        // lanes of one cell scan the same candidates in lockstep: insert a key
-        // only where it differs from the previous lane's
-        const uint32_t prev = (uint32_t)__shfl_up((int)found, 1, 64);
-        if (found != kEmpty && (lane == 0 || prev != found))
-        {
-            uint32_t h = hash_slot(found), probes = 0;
-            while (true)
-            {
-                const uint32_t old = atomicCAS(&ht[h], kEmpty, found);
-                if (old == kEmpty || old == found)
-                    break;
-                h = (h + 1u) & (kHashSize - 1u);
-                if (++probes == kHashSize)
-                {
-                    hash_full = 1;
-                    break;
-                }
-            }
-        }
     };
     constexpr uint32_t kBatch = 4;
     static_assert(kFlushEvery % kBatch == 0, "flush on batch boundaries");
@@ -550,11 +561,8 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
                         pf[q] = a.tar_f[jj[q]];
 #pragma unroll
                 for (uint32_t q = 0; q < kBatch; ++q)
-                {
-                    const uint32_t found = jj[q] != kEmpty ? test(jj[q], pf[q]) : kEmpty;
-                    if (STAGE)
-                        insert(found);
-                }
+                    if (jj[q] != kEmpty)
+                        test(jj[q], pf[q]);
                 if ((it + kBatch) % kFlushEvery == 0)
                     flush(false);
             }
@@ -567,65 +575,423 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
         if (k > b.kmax)
             atomicMax(b.needed, (unsigned long long)k);
     }
-    if (!STAGE)
-        return;
-    //----- distinct set of the block, sorted, and the ranks of the entries
-    __syncthreads(); // every wave has flushed its ring: the scratch is free
-    constexpr uint32_t per = kHashSize / kThreads;
-    uint32_t occ = 0;
-    for (uint32_t q = 0; q < per; ++q)
-        occ += ht[t * per + q] != kEmpty;
-    uint32_t total;
-    uint32_t pos = block_exclusive_scan(occ, &total);
-    if (total > b.cap || hash_full)
+}
+
+//---------------------------------------------------------------------------
+// tiled relation build: one wave per 64-particle chunk, four chunks per
+// workgroup.  The wave takes the chunk's source particles
+// one after another; its 64 lanes hold 64 consecutive candidates of each cell
+// row of the source's search box, and a ballot compacts the neighbours among
+// them, in candidate order, into the chunk's list tile in LDS (entry k of
+// chunk column c at tile[k][c]).  Sources in the same cell share their
+// candidates, which stay in registers until the cell changes.  The finished
+// tile is written out as whole 64-entry rows of the slice layout.  A tile
+// entry is 16 bits, (cell row << 12 | offset in the row), turned back into
+// the particle index when the rows are written.
+//---------------------------------------------------------------------------
+constexpr uint32_t kTileWaves = 4;
+constexpr uint32_t kTileStride = 66; // uint16 per tile row: 64 + 2 so that rows start on different banks
+constexpr uint32_t kRowCodeBits = 12;
+constexpr uint16_t kNoSlot = 0xffffu;
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t col)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)col);
+}
+__device__ __forceinline__ int lane_i32(int v, uint32_t col) { return __builtin_amdgcn_readlane(v, (int)col); }
+__device__ __forceinline__ float lane_f32(float v, uint32_t col)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)col));
+}
+__device__ __forceinline__ double lane_f64(double v, uint32_t col)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), (int)col);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), (int)col);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// v[idx] for a per-lane idx as a chain of selects kept in registers (the
+// empty asm stops the chain from being turned into a private-memory table)
+template <int R>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&v)[R], uint32_t idx)
+{
+    uint32_t out = v[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r)
     {
-        if (t == 0)
-            b.stage_count[blk] = kStageOverflow;
-        return;
+        out = idx == (uint32_t)r ? v[r] : out;
+        __asm__ volatile("" : "+v"(out));
     }
-    for (uint32_t q = 0; q < per; ++q)
-    {
-        const uint32_t v = ht[t * per + q];
-        if (v != kEmpty)
-            keys[pos++] = v;
-    }
-    uint32_t pow2 = 1;
-    while (pow2 < total)
-        pow2 <<= 1;
-    for (uint32_t q = total + t; q < pow2; q += kThreads)
-        keys[q] = kEmpty;
+    return out;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
+{
+    constexpr int R = D == 3 ? 9 : 3; // cell rows of the search box, l -> m order
+    constexpr int QMAX = 8;           // candidates of a cell kept in registers: 64 * QMAX
+    const RelationArgs &a = b.r;
+    extern __shared__ uint16_t tiles[];
+    __shared__ uint32_t any_dense;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+    const uint32_t blk = xcd_block_index();
+    const uint32_t chunk = blk * kTileWaves + wave;
+    const uint32_t i = chunk * 64u + lane;
+    const bool valid = i < a.n_src;
+    const uint32_t kmax = b.kmax;
+    uint16_t *tile = tiles + (size_t)wave * kmax * kTileStride;
+    if (t == 0)
+        any_dense = 0;
     __syncthreads();
-    for (uint32_t size = 2; size <= pow2; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+    // this lane's source particle: position, cell and its cell-row ranges
+    double xi[3] = {0.0, 0.0, 0.0};
+    int c[3] = {0, 0, 0};
+    Rec4<float> xf = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool exact_lane = false;
+    uint32_t s0[R], len[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        s0[r] = len[r] = 0;
+    if (valid)
+    {
+        load_pos<D>(a.src, i, xi);
+        cell_index<D>(a.m, xi, c);
+        xf = a.src_f[i];
+        for (int k2 = 0; k2 < D; ++k2)
+            exact_lane |= !(fabs(xi[k2]) <= a.pre_xmax);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
         {
-            for (uint32_t q = t; q < pow2; q += kThreads)
+            const int l = c[0] + (D == 3 ? r / 3 : r) - 1;
+            if (D == 3)
             {
-                const uint32_t partner = q ^ stride;
-                if (partner > q)
+                const int mm = c[1] + r % 3 - 1;
+                if (l >= 0 && l < a.m.all_cells[0] && mm >= 0 && mm < a.m.all_cells[1])
                 {
-                    const bool up = (q & size) == 0;
-                    const uint32_t x = keys[q], y = keys[partner];
-                    if ((x > y) == up)
-                    {
-                        keys[q] = y;
-                        keys[partner] = x;
-                    }
+                    const int lo2 = max(0, c[2] - 1), hi2 = min(a.m.all_cells[2], c[2] + 2);
+                    const uint32_t first =
+                        (uint32_t)(l * a.m.all_cells[1] * a.m.all_cells[2] + mm * a.m.all_cells[2] + lo2);
+                    s0[r] = a.cell_offset[first];
+                    len[r] = a.cell_offset[first + (uint32_t)(hi2 - lo2)] - s0[r];
                 }
             }
-            __syncthreads();
+            else if (l >= 0 && l < a.m.all_cells[0])
+            {
+                const int lo1 = max(0, c[1] - 1), hi1 = min(a.m.all_cells[1], c[1] + 2);
+                s0[r] = a.cell_offset[(uint32_t)(l * a.m.all_cells[1] + lo1)];
+                len[r] = a.cell_offset[(uint32_t)(l * a.m.all_cells[1] + hi1)] - s0[r];
+            }
         }
+    }
+    // an entry codes (cell row, offset in the row) in 16 bits: denser rows go
+    // to the scan kernel
+    bool dense = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        dense |= len[r] > (1u << kRowCodeBits);
+    if (__ballot(dense) && lane == 0)
+        any_dense = 1;
+    __syncthreads();
+    if (any_dense)
+    {
+        if (t == 0)
+            atomicMax(b.needed, kTooDense);
+        return; // the whole block
+    }
+    const uint32_t first_i = chunk * 64u;
+    // the flat candidate list of the current cell: entry f = 64 q + lane
+    uint32_t u_s0[R], u_p[R], u_total = 0; // wave-uniform: row starts, running offsets, length
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        u_s0[r] = u_p[r] = 0;
+    // candidate f of the flat list: cell row, offset, particle index
+    auto locate = [&](uint32_t f, uint32_t &code, uint32_t &j) {
+        uint32_t row = 0, off = f;
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+            if (f >= u_p[r])
+            {
+                row = (uint32_t)r;
+                off = f - u_p[r];
+            }
+        code = (row << kRowCodeBits) | off;
+        j = a.particle_index[pick<R>(u_s0, row) + off];
+    };
+    uint32_t cj[QMAX], ccode[QMAX];
+    float cx[QMAX], cy[QMAX], cz[QMAX];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q)
+    {
+        cj[q] = ccode[q] = 0;
+        cx[q] = cy[q] = cz[q] = 0.0f;
+    }
+    int cc0 = -1, cc1 = -1, cc2 = -1;
+    uint32_t my_count = 0;
+    // sources with candidates (most fluid particles have no wall cell near)
+    uint32_t my_total = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        my_total += len[r];
+    unsigned long long todo = __ballot(my_total > 0u);
+    while (todo)
+    {
+        const uint32_t col = (uint32_t)__ffsll((long long)todo) - 1u;
+        todo &= todo - 1ull;
+        const int q0 = lane_i32(c[0], col), q1 = lane_i32(c[1], col), q2 = D == 3 ? lane_i32(c[2], col) : 0;
+        if (q0 != cc0 || q1 != cc1 || q2 != cc2)
+        {
+            // a new cell: its candidates into registers
+            cc0 = q0;
+            cc1 = q1;
+            cc2 = q2;
+            u_total = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                u_s0[r] = lane_u32(s0[r], col);
+                u_p[r] = u_total;
+                u_total += lane_u32(len[r], col);
+            }
+            const uint32_t fill = u_total <= 64u * QMAX ? u_total : 0u;
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q)
+                if (64u * q + lane < fill)
+                    locate(64u * q + lane, ccode[q], cj[q]);
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q)
+                if (64u * q + lane < fill)
+                {
+                    const Rec4<float> p = a.tar_f[cj[q]];
+                    cx[q] = p.x;
+                    cy[q] = p.y;
+                    cz[q] = p.z;
+                }
+        }
+        const uint32_t si = first_i + col;
+        const uint32_t self = a.inner ? si : kEmpty;
+        const float sx = lane_f32(xf.x, col), sy = lane_f32(xf.y, col), sz = D == 3 ? lane_f32(xf.z, col) : 0.0f;
+        const bool s_exact = lane_u32(exact_lane ? 1u : 0u, col) != 0u;
+        const double sx0 = lane_f64(xi[0], col), sx1 = lane_f64(xi[1], col);
+        const double sx2 = D == 3 ? lane_f64(xi[2], col) : 0.0;
+        // NeighborCriterion: (inv_h * (x_i - x_j)).squaredNorm() < 4, in double
+        auto exact = [&](uint32_t j) -> bool {
+            double xj[3];
+            load_pos<D>(a.tar, j, xj);
+            const double s_0 = a.inv_h * (sx0 - xj[0]);
+            const double s_1 = a.inv_h * (sx1 - xj[1]);
+            double r2 = s_0 * s_0 + s_1 * s_1;
+            if (D == 3)
+            {
+                const double s_2 = a.inv_h * (sx2 - xj[2]);
+                r2 = r2 + s_2 * s_2;
+            }
+            return r2 < 4.0;
+        };
+        uint32_t k = 0; // neighbours of source col so far (wave-uniform)
+        if (u_total <= 64u * QMAX)
+        {
+            // every batch of the cached candidates at once, free of branches
+            // (bitwise, not short-circuit, logic) so that the batches
+            // interleave: float squared distance, neighbour and margin-shell masks
+            bool hq[QMAX], sh[QMAX];
+            unsigned long long any_shell = 0;
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q)
+            {
+                const bool act = (64u * q + lane < u_total) & (cj[q] != self);
+                const float fx = sx - cx[q], fy = sy - cy[q], fz = D == 3 ? sz - cz[q] : 0.0f;
+                // the pre-filter margin bounds the rounding of any evaluation order
+                float r2f = fx * fx;
+                r2f = fmaf(fy, fy, r2f);
+                if (D == 3)
+                    r2f = fmaf(fz, fz, r2f);
+                const bool lo = r2f < a.thr_lo, hi = r2f < a.thr_hi;
+                hq[q] = act & lo & !s_exact;
+                sh[q] = act & (s_exact | (hi & !lo));
+                any_shell |= __ballot(sh[q]);
+            }
+            if (any_shell)
+            {
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q)
+                    if (__ballot(sh[q]))
+                    {
+                        if (sh[q])
+                            hq[q] = exact(cj[q]);
+                    }
+            }
+            // compaction of the neighbours, in candidate order, into column col
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q)
+            {
+                const unsigned long long m = __ballot(hq[q]);
+                const uint32_t pos = k + lanes_below(m);
+                if (hq[q] & (pos < kmax))
+                    tile[pos * kTileStride + col] = (uint16_t)ccode[q];
+                k += (uint32_t)__popcll(m);
+            }
+        }
+        else
+        {
+            // a cell with more candidates than the registers hold: from memory
+            for (uint32_t f0 = 0; f0 < u_total; f0 += 64u)
+            {
+                const bool act = f0 + lane < u_total;
+                uint32_t code = 0, j = kEmpty;
+                bool hit = false;
+                if (act)
+                {
+                    locate(f0 + lane, code, j);
+                    if (j != self)
+                    {
+                        const Rec4<float> p = a.tar_f[j];
+                        const float fx = sx - p.x, fy = sy - p.y, fz = D == 3 ? sz - p.z : 0.0f;
+                        float r2f = fx * fx;
+                        r2f = fmaf(fy, fy, r2f);
+                        if (D == 3)
+                            r2f = fmaf(fz, fz, r2f);
+                        hit = r2f < a.thr_lo;
+                        if (s_exact || (r2f >= a.thr_lo && r2f < a.thr_hi))
+                            hit = exact(j);
+                    }
+                }
+                const unsigned long long mask = __ballot(hit);
+                const uint32_t pos = k + lanes_below(mask);
+                if (hit && pos < kmax)
+                    tile[pos * kTileStride + col] = (uint16_t)code;
+                k += (uint32_t)__popcll(mask);
+            }
+        }
+        if (lane == col)
+            my_count = k;
+    }
+    const uint32_t mine = valid ? min(my_count, kmax) : 0u;
+    if (valid)
+    {
+        a.neighbor_size[i] = mine;
+        if (my_count > kmax)
+            atomicMax(b.needed, (unsigned long long)my_count);
+    }
+    // rows out: decode this lane's entries (its own source's rows) and write
+    // the slice rows
+    const size_t row0 = (size_t)chunk * kmax * 64u + lane;
+    const uint32_t rows = wave_max_u32(mine);
+    constexpr uint32_t kOutBatch = 8; // index gathers in flight per lane
+    for (uint32_t r0 = 0; r0 < rows; r0 += kOutBatch)
+    {
+        uint32_t jj[kOutBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kOutBatch; ++u)
+        {
+            const uint32_t r = r0 + u;
+            jj[u] = kEmpty;
+            if (r < mine)
+            {
+                const uint32_t e = tile[r * kTileStride + lane];
+                const uint32_t row = e >> kRowCodeBits, off = e & ((1u << kRowCodeBits) - 1u);
+                jj[u] = a.particle_index[pick<R>(s0, row) + off];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kOutBatch; ++u)
+            if (r0 + u < rows)
+                a.slice[row0 + (size_t)(r0 + u) * 64u] = jj[u];
+    }
+}
+
+//---------------------------------------------------------------------------
+// staging sets of the inner fluid relation, from its finished slices: for each
+// block of kStageBlock source particles the sorted distinct neighbours
+// (stage_keys, stage_count) and every entry as a 16-bit rank into them
+// (local_slice), or kStageOverflow when the set exceeds the capacity.  A hash
+// set in LDS collects the distinct indices, a bitonic sort fixes their order
+// (so the result does not depend on insertion order), a rank table maps each
+// hash slot to the sorted position.
+//---------------------------------------------------------------------------
+struct StageArgs
+{
+    uint32_t n_src, cap;
+    const uint32_t *neighbor_size, *chunk_offset, *slice;
+    uint32_t *stage_count, *stage_keys;
+    uint16_t *local_slice;
+};
+
+__global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
+{
+    __shared__ uint32_t ht[kHashSize];
+    __shared__ uint32_t keys[kStageMaxCap];
+    __shared__ uint16_t rank[kHashSize];
+    __shared__ uint32_t hash_full;
+    const uint32_t t = threadIdx.x;
+    const uint32_t blk = xcd_block_index();
+    const uint32_t i = blk * kThreads + t;
+    const bool valid = i < s.n_src;
+    for (uint32_t k = t; k < kHashSize; k += kThreads)
+        ht[k] = kEmpty;
+    if (t == 0)
+        hash_full = 0;
+    __syncthreads();
+    const uint32_t count = valid ? s.neighbor_size[i] : 0u;
+    const size_t row0 = valid ? (size_t)s.chunk_offset[i >> 6] + (i & 63u) : 0;
+    constexpr uint32_t kInBatch = 8; // row reads in flight per lane
+    for (uint32_t r0 = 0; r0 < count; r0 += kInBatch)
+    {
+        uint32_t jj[kInBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kInBatch; ++u)
+            jj[u] = r0 + u < count ? s.slice[row0 + (size_t)(r0 + u) * 64u] : kEmpty;
+#pragma unroll
+        for (uint32_t u = 0; u < kInBatch; ++u)
+        {
+            if (jj[u] == kEmpty)
+                continue;
+            uint32_t h = hash_slot(jj[u]), probes = 0;
+            while (true)
+            {
+                const uint32_t old = atomicCAS(&ht[h], kEmpty, jj[u]);
+                if (old == kEmpty || old == jj[u])
+                    break;
+                h = (h + 1u) & (kHashSize - 1u);
+                if (++probes == kHashSize)
+                {
+                    hash_full = 1;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t total = 0;
+    if (!block_collect_sorted(ht, keys, s.cap, hash_full, &total))
+    {
+        if (t == 0)
+            s.stage_count[blk] = kStageOverflow;
+        return;
+    }
     for (uint32_t r = t; r < total; r += kThreads)
     {
         rank[hash_find(ht, keys[r])] = (uint16_t)r;
-        b.stage_keys[(size_t)blk * b.cap + r] = keys[r];
+        s.stage_keys[(size_t)blk * s.cap + r] = keys[r];
     }
     __syncthreads();
-    // every entry of this lane was written by this lane (ring flush or direct)
-    const uint32_t mine = valid ? min(k, b.kmax) : 0u;
-    for (uint32_t q = 0; q < mine; ++q)
-        b.local_slice[row0 + (size_t)q * 64u] = rank[hash_find(ht, a.slice[row0 + (size_t)q * 64u])];
+    for (uint32_t r0 = 0; r0 < count; r0 += kInBatch)
+    {
+        uint32_t jj[kInBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kInBatch; ++u)
+            jj[u] = r0 + u < count ? s.slice[row0 + (size_t)(r0 + u) * 64u] : kEmpty;
+#pragma unroll
+        for (uint32_t u = 0; u < kInBatch; ++u)
+            if (jj[u] != kEmpty)
+                s.local_slice[row0 + (size_t)(r0 + u) * 64u] = rank[hash_find(ht, jj[u])];
+    }
     if (t == 0)
-        b.stage_count[blk] = total;
+        s.stage_count[blk] = total;
 }
 
 __global__ void k_chunk_stride(uint32_t *chunk, uint32_t nchunks, uint32_t stride)
@@ -814,9 +1180,23 @@ sphx_status relation_to_csr(sphx_relation *rel)
     return SPHX_OK;
 }
 
-// UpdateRelation in one scan: fixed kmax rows per chunk; *needed > kmax on return
-// asks the caller to grow kmax and run again
-sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed)
+template <int D>
+static sphx_status launch_tile(sphx_context *ctx, const BuildArgs &b, uint32_t nchunks, size_t lds)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_relation_tile<D>);
+    if (lds > 65536)
+        SPHX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_relation_tile<D>), dim3(grid_for(nchunks, kTileWaves)), dim3(kThreads), (unsigned)lds,
+                       ctx->stream, b);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+// UpdateRelation in one pass: fixed kmax rows per chunk; *needed > kmax on
+// return asks the caller to grow kmax and run again, *needed == kTooDense to
+// run again with scan = true.  The tiled kernel runs when its LDS tile for
+// kmax rows fits the device, the per-lane scan kernel otherwise.
+sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed, bool scan)
 {
     uint32_t n = rel->source->n;
     sphx_context *ctx = rel->source->ctx;
@@ -829,26 +1209,40 @@ sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long
     b.r = relation_args(rel);
     b.kmax = kmax;
     b.needed = needed;
-    b.cap = rel->stage_cap;
-    b.stage_count = rel->stage_count;
-    b.stage_keys = rel->stage_keys;
-    b.local_slice = rel->local_slice;
-    ProfileScope prof(ctx, "relation_build");
+    const size_t lds = (size_t)kTileWaves * kmax * kTileStride * sizeof(uint16_t);
+    if (!scan && lds <= ctx->max_lds)
+    {
+        ProfileScope prof(ctx, "relation_build");
+        return rel->source->D == 2 ? launch_tile<2>(ctx, b, nchunks, lds) : launch_tile<3>(ctx, b, nchunks, lds);
+    }
+    ProfileScope prof(ctx, "relation_scan");
     dim3 g(grid_for(n, kThreads)), blk(kThreads);
     if (rel->source->D == 2)
-    {
-        if (rel->staged)
-            hipLaunchKernelGGL((k_relation_build<2, true>), g, blk, 0, ctx->stream, b);
-        else
-            hipLaunchKernelGGL((k_relation_build<2, false>), g, blk, 0, ctx->stream, b);
-    }
+        hipLaunchKernelGGL((k_relation_build<2>), g, blk, 0, ctx->stream, b);
     else
-    {
-        if (rel->staged)
-            hipLaunchKernelGGL((k_relation_build<3, true>), g, blk, 0, ctx->stream, b);
-        else
-            hipLaunchKernelGGL((k_relation_build<3, false>), g, blk, 0, ctx->stream, b);
-    }
+        hipLaunchKernelGGL((k_relation_build<3>), g, blk, 0, ctx->stream, b);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+// staging sets of a staged relation from its current slices
+sphx_status relation_stage(sphx_relation *rel)
+{
+    uint32_t n = rel->source->n;
+    sphx_context *ctx = rel->source->ctx;
+    if (n == 0 || !rel->staged)
+        return SPHX_OK;
+    StageArgs a;
+    a.n_src = n;
+    a.cap = rel->stage_cap;
+    a.neighbor_size = rel->neighbor_size;
+    a.chunk_offset = rel->chunk_offset;
+    a.slice = rel->slice;
+    a.stage_count = rel->stage_count;
+    a.stage_keys = rel->stage_keys;
+    a.local_slice = rel->local_slice;
+    ProfileScope prof(ctx, "relation_stage");
+    hipLaunchKernelGGL(k_stage_sets, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

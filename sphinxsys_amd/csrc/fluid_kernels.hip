@@ -1,12 +1,15 @@
 // fluid_kernels.hip -- the weakly-compressible fluid step on gfx950.
 //
-// One thread per fluid particle; each thread walks its own CSR neighbour list
-// (inner, then each wall contact), gathers the neighbour's (x, y, z, Vol)
-// record and pressure or velocity, evaluates the tabulated Wendland C2
-// gradient on the fly (cubic per table interval, table in LDS) and fuses the
-// reference's initialize / interact / update stages of one CK dynamics where
-// the data dependencies allow it.  The gather stencil has no dense
-// contraction, so MFMA is not used; the bound is HBM/L2 traffic and VALU.
+// One thread per fluid particle; each thread walks its own neighbour list
+// (inner, then each wall contact) in the 64-wide slice layout (entry k of the
+// 64 particles of a chunk is one contiguous row), gathers the neighbour's
+// (x, y, z, Vol) record and pressure or velocity -- from LDS where the
+// relation staged the block's distinct neighbours, from global memory
+// otherwise -- evaluates the tabulated Wendland C2 gradient on the fly (cubic
+// per table interval, table in LDS) and fuses the reference's initialize /
+// interact / update stages of one CK dynamics where the data dependencies
+// allow it.  The gather stencil has no dense contraction, so MFMA is not
+// used; the bound is HBM/L2 traffic and VALU.
 //
 // Reference: shared_ck/particle_dynamics/fluid_dynamics/acoustic_step_1st_half.hpp:66-180,
 // acoustic_step_2nd_half.hpp:33-137, riemann_solver/riemann_solver_ck.hpp:17-69,

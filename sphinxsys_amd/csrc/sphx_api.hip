@@ -771,7 +771,8 @@ sphx_status sphx_update_relation(sphx_relation *r)
     const size_t n = r->source->n;
     const size_t nchunks = (n + 63) / 64;
     uint32_t kmax = r->kmax ? r->kmax : (r->source->D == 3 ? 96u : 32u);
-    for (int attempt = 0; attempt < 4; ++attempt)
+    bool scan = false, done = false;
+    for (int attempt = 0; attempt < 5 && !done; ++attempt)
     {
         const size_t need = std::max<size_t>(nchunks * kmax * 64, 64);
         if (need >= (1ull << 32))
@@ -794,12 +795,20 @@ sphx_status sphx_update_relation(sphx_relation *r)
             r->slice_capacity = need;
         }
         if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_NEEDED, 1, 0ull)) ||
-            (s = relation_build(r, kmax, ctx->scalars_d + SLOT_NEEDED)) ||
+            (s = relation_build(r, kmax, ctx->scalars_d + SLOT_NEEDED, scan)) ||
             (s = read_scalars(ctx, SLOT_NEEDED, 1)))
             return s;
         const unsigned long long needed = ctx->scalars_h[SLOT_NEEDED];
         if (needed <= kmax)
+        {
+            done = true;
             break;
+        }
+        if (needed >= kTooDense)
+        {
+            scan = true; // a cell row too dense for the tiled build's row code
+            continue;
+        }
         if (needed > 60000)
         {
             set_error("a particle has more than 60000 neighbours: positions collapsed");
@@ -807,8 +816,15 @@ sphx_status sphx_update_relation(sphx_relation *r)
         }
         kmax = (uint32_t)((needed + needed / 4 + 7) / 8 * 8);
     }
+    if (!done)
+    {
+        set_error("neighbour lists did not fit after growing the slice width");
+        return SPHX_E_CAPACITY;
+    }
     r->kmax = kmax;
     r->slice_total = nchunks * kmax * 64;
+    if ((s = relation_stage(r)))
+        return s;
     // CSR offsets of the reference layout (NeighborSize scan), for reads
     if ((s = exclusive_scan_u32(ctx, r->neighbor_size, r->particle_offset, n + 1, nullptr)))
         return s;

@@ -33,7 +33,10 @@ struct Scratch
 constexpr int kScalarSlots = 64;
 constexpr uint32_t kStageBlock = 256;          // source particles per staging block
 constexpr uint32_t kStageOverflow = 0xffffffffu;
-constexpr uint32_t kStageMaxCap = 2048;        // largest staged set the localize kernel sorts
+constexpr uint32_t kStageMaxCap = 2048;        // largest staged set the staging kernel sorts
+// reported through the relation build's `needed` slot when a cell row holds
+// more candidates than the tiled build can code: rebuild with the scan kernel
+constexpr unsigned long long kTooDense = 1ull << 40;
 enum ScalarSlot
 {
     SLOT_MAX_A = 0,      // ordered-bits max accumulators (u64)
@@ -143,7 +146,10 @@ sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *
 sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
 sphx_status cell_linked_list_update(sphx_cell_linked_list *cll);
 sphx_status relation_to_csr(sphx_relation *rel);
-sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed);
+// scan = true forces the per-lane scan kernel (no LDS tile)
+sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed, bool scan);
+// the LDS staging sets of a staged relation, from its finished slices
+sphx_status relation_stage(sphx_relation *rel);
 sphx_status morton_sequence(sphx_body *b, sphx_cell_linked_list *cll);
 sphx_status gather_u32(sphx_context *ctx, uint32_t *field, const uint32_t *perm, size_t n);
 sphx_status gather_real(sphx_body *b, void **field, const uint32_t *perm);

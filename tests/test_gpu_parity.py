@@ -129,10 +129,55 @@ def test_relations_bit_exact(hip, oracle, dim):
     pr.close()
 
 
-def perturbed_pair(hip, oracle, dim, precision=capi.FP64, seed=1):
+def shuffled(pos, window, seed):
+    """Particle order shuffled within consecutive windows (window 0: whole
+    array), so that a 256-particle block has neighbours spread over many cells."""
+    rng = np.random.default_rng(seed)
+    idx = np.arange(len(pos))
+    w = window or len(pos)
+    for s in range(0, len(pos), w):
+        rng.shuffle(idx[s:s + w])
+    return pos[idx]
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+@pytest.mark.parametrize("window", [1024, 0])
+def test_relations_bit_exact_shuffled(hip, oracle, precision, window):
+    """Unsorted particle orders: the staged (MIXED) build meets blocks whose
+    distinct-neighbour set exceeds the staging capacity (window 1024) or the
+    LDS hash (full shuffle); the lists must stay bit-exact."""
+    dp = 0.02
+    fluid = shuffled(jittered_lattice(3, 24, dp, 17), window, 5)
+    pr = Pair(hip, oracle, fluid, dp, precision=precision)
+    for g, o in zip(pr.gpu["inner"].read(), pr.cpu["inner"].read()):
+        assert np.array_equal(g, o)
+    pr.close()
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_relations_dense_cluster(hip, oracle, precision):
+    """A cluster of 4500 particles inside one cell (every pair within the
+    cut-off) next to a regular lattice: lists of 4499 entries, beyond the
+    tiled build's row code and LDS tile, built by the scan kernel."""
+    dp = 0.02
+    rng = np.random.default_rng(31)
+    lattice = jittered_lattice(2, 40, dp, 30)
+    cluster = 0.4 + rng.uniform(0.0, 0.3 * dp, size=(4500, 2))
+    fluid = np.concatenate([lattice, cluster])
+    pr = Pair(hip, oracle, fluid, dp, precision=precision)
+    for g, o in zip(pr.gpu["inner"].read(), pr.cpu["inner"].read()):
+        assert np.array_equal(g, o)
+    size = pr.gpu["inner"].read()[0]
+    assert size.max() >= 4499
+    pr.close()
+
+
+def perturbed_pair(hip, oracle, dim, precision=capi.FP64, seed=1, window=None):
     dp = 0.02
     m = 40 if dim == 2 else 16
     fluid = jittered_lattice(dim, m, dp, seed)
+    if window is not None:
+        fluid = shuffled(fluid, window, seed + 20)
     # a wall slab below the fluid
     slab = jittered_lattice(dim, m, dp, seed + 1, jitter=0.0)
     wall = slab[slab[:, -1] < 3 * dp].copy()
@@ -191,12 +236,13 @@ def test_acoustic_step_fp64(hip, oracle, dim, riemann):
     pr.close()
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_acoustic_step_mixed_against_fp64_oracle(hip, oracle, dim):
+@pytest.mark.parametrize("dim,window", [(2, None), (3, None), (3, 1024), (3, 0)])
+def test_acoustic_step_mixed_against_fp64_oracle(hip, oracle, dim, window):
     """FP32 pair interactions: per-step force and compression rate within 2e-4
     relative L2 of the FP64 oracle (float rounding of positions at ~1e-7 of the
-    domain and of the pair sums)."""
-    pr = perturbed_pair(hip, oracle, dim, precision=capi.MIXED)
+    domain and of the pair sums).  Shuffled orders send blocks through the
+    unstaged (global gather) path of the passes."""
+    pr = perturbed_pair(hip, oracle, dim, precision=capi.MIXED, window=window)
     dt = 2e-5
     pr.both(lambda s: ck.CompressionSummation(s["inner"], *s["walls"]).exec())
     g, c = pr.read("CompressionSummation")
