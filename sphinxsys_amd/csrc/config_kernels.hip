@@ -343,11 +343,7 @@ __global__ void k_slice_to_csr(const uint32_t *count, const uint32_t *chunk, con
 }
 
 //---------------------------------------------------------------------------
-// staging sets: for each block of kStageBlock source particles, the sorted
-// distinct neighbours and the 16-bit rank of every neighbour entry in it.
-// A hash set in LDS collects the distinct indices, a bitonic sort fixes their
-// order (so the result does not depend on insertion order), a second hash
-// table maps an index to its rank.
+// LDS hash set of particle indices (linear probing), used for the staging sets
 //---------------------------------------------------------------------------
 constexpr uint32_t kHashSize = 4096;
 constexpr uint32_t kEmpty = 0xffffffffu;
@@ -362,12 +358,12 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t *ht, uint32_t key)
     return s;
 }
 
-// Collects the occupied slots of the block's hash set into keys[0, total) and
-// sorts them ascending (bitonic, padded to a power of two with kEmpty).
-// Returns false, with keys untouched, when the set overflowed the hash or has
-// more than `cap` entries.  Called by all threads of a 256-thread block.
-__device__ bool block_collect_sorted(const uint32_t *ht, uint32_t *keys, uint32_t cap, uint32_t hash_full,
-                                     uint32_t *total_out)
+// Collects the occupied slots of the block's hash set, in slot order, into
+// keys[0, total).  Returns false, with keys untouched, when the set overflowed
+// the hash or has more than `cap` entries.  Called by all threads of a
+// 256-thread block.
+__device__ bool block_collect(const uint32_t *ht, uint32_t *keys, uint32_t cap, uint32_t hash_full,
+                              uint32_t *total_out)
 {
     const uint32_t t = threadIdx.x;
     constexpr uint32_t per = kHashSize / kThreads;
@@ -385,31 +381,7 @@ __device__ bool block_collect_sorted(const uint32_t *ht, uint32_t *keys, uint32_
         if (v != kEmpty)
             keys[pos++] = v;
     }
-    uint32_t pow2 = 1;
-    while (pow2 < total)
-        pow2 <<= 1;
-    for (uint32_t q = total + t; q < pow2; q += kThreads)
-        keys[q] = kEmpty;
     __syncthreads();
-    for (uint32_t size = 2; size <= pow2; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
-        {
-            for (uint32_t q = t; q < pow2; q += kThreads)
-            {
-                const uint32_t partner = q ^ stride;
-                if (partner > q)
-                {
-                    const bool up = (q & size) == 0;
-                    const uint32_t x = keys[q], y = keys[partner];
-                    if ((x > y) == up)
-                    {
-                        keys[q] = y;
-                        keys[partner] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
     return true;
 }
 
@@ -593,6 +565,7 @@ constexpr uint32_t kTileWaves = 4;
 constexpr uint32_t kTileStride = 66; // uint16 per tile row: 64 + 2 so that rows start on different banks
 constexpr uint32_t kRowCodeBits = 12;
 constexpr uint16_t kNoSlot = 0xffffu;
+constexpr float kFarAway = 1.0e30f; // position of an unused candidate slot (squared distance overflows to inf)
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t col)
 {
@@ -610,10 +583,12 @@ __device__ __forceinline__ double lane_f64(double v, uint32_t col)
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), (int)col);
     return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
 }
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask)
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask, uint32_t add = 0u)
 {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, add));
 }
+// lane mask of a per-lane condition (the compare result itself, no int round trip)
+__device__ __forceinline__ unsigned long long lane_mask(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
 // v[idx] for a per-lane idx as a chain of selects kept in registers (the
 // empty asm stops the chain from being turned into a private-memory table)
@@ -764,20 +739,26 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                     locate(64u * q + lane, ccode[q], cj[q]);
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
+            {
+                // slots past the list hold a far position: never a neighbour
+                Rec4<float> p = {kFarAway, kFarAway, kFarAway, 0.0f};
                 if (64u * q + lane < fill)
-                {
-                    const Rec4<float> p = a.tar_f[cj[q]];
-                    cx[q] = p.x;
-                    cy[q] = p.y;
-                    cz[q] = p.z;
-                }
+                    p = a.tar_f[cj[q]];
+                cx[q] = p.x;
+                cy[q] = p.y;
+                cz[q] = p.z;
+            }
         }
         const uint32_t si = first_i + col;
         const uint32_t self = a.inner ? si : kEmpty;
         const float sx = lane_f32(xf.x, col), sy = lane_f32(xf.y, col), sz = D == 3 ? lane_f32(xf.z, col) : 0.0f;
         const bool s_exact = lane_u32(exact_lane ? 1u : 0u, col) != 0u;
-        const double sx0 = lane_f64(xi[0], col), sx1 = lane_f64(xi[1], col);
-        const double sx2 = D == 3 ? lane_f64(xi[2], col) : 0.0;
+        double sx0 = 0.0, sx1 = 0.0, sx2 = 0.0; // the source position, read when a shell needs it
+        auto source_position = [&]() {
+            sx0 = lane_f64(xi[0], col);
+            sx1 = lane_f64(xi[1], col);
+            sx2 = D == 3 ? lane_f64(xi[2], col) : 0.0;
+        };
         // NeighborCriterion: (inv_h * (x_i - x_j)).squaredNorm() < 4, in double
         auto exact = [&](uint32_t j) -> bool {
             double xj[3];
@@ -803,23 +784,25 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
             {
-                const bool act = (64u * q + lane < u_total) & (cj[q] != self);
+                const bool other = cj[q] != self;
                 const float fx = sx - cx[q], fy = sy - cy[q], fz = D == 3 ? sz - cz[q] : 0.0f;
                 // the pre-filter margin bounds the rounding of any evaluation order
                 float r2f = fx * fx;
                 r2f = fmaf(fy, fy, r2f);
                 if (D == 3)
                     r2f = fmaf(fz, fz, r2f);
-                const bool lo = r2f < a.thr_lo, hi = r2f < a.thr_hi;
-                hq[q] = act & lo & !s_exact;
-                sh[q] = act & (s_exact | (hi & !lo));
-                any_shell |= __ballot(sh[q]);
+                // exact lanes: every listed candidate goes to the shell test
+                const bool lo = r2f < a.thr_lo, hi = r2f < (s_exact ? kFarAway : a.thr_hi);
+                hq[q] = other & lo & !s_exact;
+                sh[q] = other & hi & !(lo & !s_exact);
+                any_shell |= lane_mask(sh[q]);
             }
             if (any_shell)
             {
+                source_position();
 #pragma unroll
                 for (int q = 0; q < QMAX; ++q)
-                    if (__ballot(sh[q]))
+                    if (lane_mask(sh[q]))
                     {
                         if (sh[q])
                             hq[q] = exact(cj[q]);
@@ -829,8 +812,8 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
             {
-                const unsigned long long m = __ballot(hq[q]);
-                const uint32_t pos = k + lanes_below(m);
+                const unsigned long long m = lane_mask(hq[q]);
+                const uint32_t pos = lanes_below(m, k);
                 if (hq[q] & (pos < kmax))
                     tile[pos * kTileStride + col] = (uint16_t)ccode[q];
                 k += (uint32_t)__popcll(m);
@@ -839,6 +822,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
         else
         {
             // a cell with more candidates than the registers hold: from memory
+            source_position();
             for (uint32_t f0 = 0; f0 < u_total; f0 += 64u)
             {
                 const bool act = f0 + lane < u_total;
@@ -906,12 +890,14 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 
 //---------------------------------------------------------------------------
 // staging sets of the inner fluid relation, from its finished slices: for each
-// block of kStageBlock source particles the sorted distinct neighbours
-// (stage_keys, stage_count) and every entry as a 16-bit rank into them
-// (local_slice), or kStageOverflow when the set exceeds the capacity.  A hash
-// set in LDS collects the distinct indices, a bitonic sort fixes their order
-// (so the result does not depend on insertion order), a rank table maps each
-// hash slot to the sorted position.
+// block of kStageBlock source particles the distinct neighbours (stage_keys,
+// stage_count) and every entry as a 16-bit rank into them (local_slice), or
+// kStageOverflow when the set exceeds the capacity.  A hash set in LDS
+// collects the distinct indices and a rank table maps each occupied slot to
+// its position in slot order.  Where keys collide the rank assignment depends
+// on the order of insertion; the passes use a rank only as an LDS address, so
+// their results do not.  (Sorting the set would make the staging loads of the
+// passes read nearby particles together, but costs more than it saves.)
 //---------------------------------------------------------------------------
 struct StageArgs
 {
@@ -967,7 +953,7 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
     }
     __syncthreads();
     uint32_t total = 0;
-    if (!block_collect_sorted(ht, keys, s.cap, hash_full, &total))
+    if (!block_collect(ht, keys, s.cap, hash_full, &total))
     {
         if (t == 0)
             s.stage_count[blk] = kStageOverflow;

@@ -334,7 +334,7 @@ __device__ __forceinline__ T p_jump(const FluidArgs<T> &a, T u)
 //---------------------------------------------------------------------------
 // pass 2: inner + wall interact of AcousticStep2ndHalf, its update
 // (C += dt/2 dC, rho = C rho0) and, fused, the per-particle signal speed of
-// AcousticTimeStepCK for the next sub-step, max-reduced into *max_slot.
+// AcousticTimeStepCK for the next sub-step, max-reduced into the max lanes at max_slot.
 //---------------------------------------------------------------------------
 template <int D, int P, bool RIEMANN, bool FUSE_CFL>
 __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d,
@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
             double b = wave_red[0];
             for (int w = 1; w < kThreads / 64; ++w)
                 b = fmax(b, wave_red[w]);
-            atomicMax(max_slot, ordered_bits(b));
+            atomic_max_spread(max_slot, b);
         }
     }
 }
@@ -495,26 +495,52 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
     using T = typename Prec<P>::T;
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    extern __shared__ __align__(32) unsigned char staged_lds[];
     stage_poly(poly_in, a.poly, false);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, false);
+    const uint32_t blk = xcd_block_index();
+    const uint32_t i = blk * kThreads + threadIdx.x;
+    // staged neighbour records: (x, y, z, Vol_ref) of the block's distinct neighbours
+    Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
+    const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
+    const bool staged = n_staged != kStageOverflow;
+    if (staged)
+    {
+        const uint32_t *keys = a.stage_keys + (size_t)blk * a.stage_cap;
+        for (uint32_t r = threadIdx.x; r < n_staged; r += kThreads)
+        {
+            const uint32_t j = keys[r];
+            Rec4<T> rec = a.posvol[j];
+            if (D == 3)
+                rec.w = a.Vol_ref[j];
+            else
+                rec.z = a.Vol_ref[j];
+            s_pv[r] = rec;
+        }
+    }
     __syncthreads();
-    uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
     if (i >= a.n)
         return;
     const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
     const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
     T s = a.poly->w0 * a.Vol_ref[i];
-    const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
-    for (uint32_t k = 0; k < nc; ++k)
-    {
-        const uint32_t j = a.slice[nb + (k << 6)];
-        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+    // W_ij Vref_j (pj.vol carries Vref_j)
+    auto pair = [&](const PV<T, D> &pj) {
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
         const T q = sqrt(r2) * inv_h;
-        s += eval_poly(poly_in, q, inv_dq, dq) * a.Vol_ref[j];
-    }
+        s += eval_poly(poly_in, q, inv_dq, dq) * pj.vol;
+    };
+    const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+    if (staged)
+        for_each_neighbor(a.local_slice, nb, nc, [&](uint32_t r) { pair(load_pv<T, D>(s_pv, r)); });
+    else
+        for_each_neighbor(a.slice, nb, nc, [&](uint32_t j) {
+            PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+            pj.vol = a.Vol_ref[j];
+            pair(pj);
+        });
     for (int k = 0; k < a.n_wall; ++k)
     {
         const WallArgs<T> &w = a.wall[k];
@@ -669,7 +695,7 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
         double b = red[0];
         for (int w = 1; w < kThreads / 64; ++w)
             b = fmax(b, red[w]);
-        atomicMax(slot, ordered_bits(b));
+        atomic_max_spread(slot, b);
     }
 }
 
@@ -1018,7 +1044,9 @@ struct SummationL
         FluidArgs<T> a = fluid_args<T>(b);
         attach_relations(a, inner, c, nc);
         ProfileScope prof(b->ctx, "compression_summation");
-        hipLaunchKernelGGL((k_summation<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
+        const size_t lds = a.stage_count ? (size_t)a.stage_cap * sizeof(Rec4<T>) : 0;
+        allow_lds(k_summation<D, P>, lds);
+        hipLaunchKernelGGL((k_summation<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), lds,
                            b->ctx->stream, a);
         LAUNCH_CHECK();
         return SPHX_OK;
@@ -1130,7 +1158,7 @@ sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int
                               bool run_init, bool fuse_cfl)
 {
     sphx_body *b = inner->source;
-    unsigned long long *slot = fuse_cfl ? b->ctx->scalars_d + SLOT_MAX_A : nullptr;
+    unsigned long long *slot = fuse_cfl ? b->ctx->scalars_d + SLOT_MAX : nullptr;
     return dispatch<Acoustic2L>(b, inner, w, nw, riemann, dt, run_init, slot);
 }
 

@@ -975,12 +975,20 @@ sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *con
     return acoustic_2nd_half(inner, walls, n, riemann, dt, true, false);
 }
 
+// the max lanes: reset before a max reduction, combined when read
+static sphx_status reset_max(sphx_context *ctx)
+{
+    return fill_u64(ctx, ctx->scalars_d + SLOT_MAX, (size_t)kMaxLanes * kMaxStride, kOrderedLowest);
+}
 static sphx_status read_max(sphx_context *ctx, double *out)
 {
-    sphx_status s = read_scalars(ctx, SLOT_MAX_A, 1);
+    sphx_status s = read_scalars(ctx, SLOT_MAX, kMaxLanes * kMaxStride);
     if (s)
         return s;
-    *out = from_ordered_bits(ctx->scalars_h[SLOT_MAX_A]);
+    unsigned long long m = 0;
+    for (int k = 0; k < kMaxLanes; ++k)
+        m = std::max(m, ctx->scalars_h[SLOT_MAX + k * kMaxStride]);
+    *out = from_ordered_bits(m);
     return SPHX_OK;
 }
 
@@ -991,8 +999,8 @@ sphx_status sphx_acoustic_time_step(sphx_body *b, double cfl, double *dt)
     SPHX_HIP(hipSetDevice(b->ctx->device));
     sphx_context *ctx = b->ctx;
     sphx_status s;
-    if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
-        (s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX_A)))
+    if ((s = reset_max(ctx)) ||
+        (s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX)))
         return s;
     double m = 0;
     if ((s = read_max(ctx, &m)))
@@ -1009,8 +1017,8 @@ sphx_status sphx_advection_time_step(sphx_body *b, double u_ref, double cfl, dou
     SPHX_HIP(hipSetDevice(b->ctx->device));
     sphx_context *ctx = b->ctx;
     sphx_status s;
-    if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
-        (s = advection_speed_max(b, ctx->scalars_d + SLOT_MAX_A)))
+    if ((s = reset_max(ctx)) ||
+        (s = advection_speed_max(b, ctx->scalars_d + SLOT_MAX)))
         return s;
     double m = 0;
     if ((s = read_max(ctx, &m)))
@@ -1040,8 +1048,8 @@ sphx_status sphx_upper_front_in_axis(sphx_body *b, int axis, double *front)
     SPHX_HIP(hipSetDevice(b->ctx->device));
     sphx_context *ctx = b->ctx;
     sphx_status s;
-    if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
-        (s = axis_max(b, axis, ctx->scalars_d + SLOT_MAX_A)))
+    if ((s = reset_max(ctx)) ||
+        (s = axis_max(b, axis, ctx->scalars_d + SLOT_MAX)))
         return s;
     return read_max(ctx, front);
 }
@@ -1095,7 +1103,7 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     while (relaxation_time < adv_dt)
     {
         if ((s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, dt, true)) ||
-            (s = fill_u64(ctx, ctx->scalars_d + SLOT_MAX_A, 1, kOrderedLowest)) ||
+            (s = reset_max(ctx)) ||
             (s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, dt, false, true)))
             return s;
         relaxation_time += dt;

@@ -104,6 +104,17 @@ __host__ __device__ inline double from_ordered_bits(unsigned long long u)
     return d;
 }
 
+// Max reductions accumulate with atomicMax on the ordered bits of a double.
+// Device-scope atomics on one address serialise in memory, so each workgroup
+// uses one of kMaxLanes accumulators, kMaxStride u64 (one 128-B line) apart,
+// and the reader takes the maximum of the lanes.
+constexpr int kMaxLanes = 32;
+constexpr int kMaxStride = 16;
+__device__ __forceinline__ void atomic_max_spread(unsigned long long *base, double v)
+{
+    atomicMax(base + (blockIdx.x % kMaxLanes) * kMaxStride, ordered_bits(v));
+}
+
 // wave64 reductions with DPP-free shuffles
 __device__ __forceinline__ double wave_max(double v)
 {

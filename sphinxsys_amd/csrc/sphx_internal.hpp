@@ -30,7 +30,7 @@ struct Scratch
     ~Scratch();
 };
 
-constexpr int kScalarSlots = 64;
+constexpr int kScalarSlots = 1024;
 constexpr uint32_t kStageBlock = 256;          // source particles per staging block
 constexpr uint32_t kStageOverflow = 0xffffffffu;
 constexpr uint32_t kStageMaxCap = 2048;        // largest staged set the staging kernel sorts
@@ -39,12 +39,11 @@ constexpr uint32_t kStageMaxCap = 2048;        // largest staged set the staging
 constexpr unsigned long long kTooDense = 1ull << 40;
 enum ScalarSlot
 {
-    SLOT_MAX_A = 0,      // ordered-bits max accumulators (u64)
-    SLOT_MAX_B = 1,
     SLOT_TOTAL = 2,      // u32 totals returned by scans (stored as u64)
     SLOT_DT = 3,         // double: acoustic dt of the current sub-step
     SLOT_NEEDED = 4,     // u64: neighbour count beyond the slice width
-    SLOT_SUM_BASE = 8    // double partial sums region start (reduce_sum final)
+    SLOT_SUM_BASE = 8,   // double partial sums region start (reduce_sum final)
+    SLOT_MAX = 512       // kMaxLanes ordered-bits max accumulators, kMaxStride apart
 };
 } // namespace sphx
 
@@ -123,7 +122,7 @@ struct sphx_relation
     uint32_t kmax = 0;                    // rows per 64-particle chunk of the slices
     bool csr_valid = false;
     // LDS staging of the inner relation: per block of kStageBlock source
-    // particles, the sorted set of distinct neighbours (stage_keys[b * stage_cap + r],
+    // particles, the set of distinct neighbours (stage_keys[b * stage_cap + r],
     // stage_count[b] of them, or kStageOverflow) and the neighbour entries as
     // 16-bit ranks into that set (local_slice, same slice layout)
     bool staged = false;
@@ -169,7 +168,7 @@ sphx_status compression_summation(sphx_relation *inner, sphx_relation *const *c,
 sphx_status acoustic_1st_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                               double dt, bool fuse_init_2nd);
 // run_init: launch the 2nd half's initialize (false when pass 1 fused it);
-// fuse_cfl: max-reduce the next sub-step's signal speed into SLOT_MAX_A
+// fuse_cfl: max-reduce the next sub-step's signal speed into the SLOT_MAX lanes
 sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                               double dt, bool run_init, bool fuse_cfl);
 sphx_status acoustic_signal_max(sphx_body *b, unsigned long long *slot);
