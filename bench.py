@@ -48,8 +48,26 @@ STEP_BYTES = {"mixed": 340.0, "fp64": 545.0}
 FLOP_PER_PAIR = 125.0
 
 
+PROFILE = os.path.join(ROOT, "profiles", "dambreak3d_8M_mixed.md")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def profiled_traffic(stage):
+    """HBM bytes per launch of `stage` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE on this workload (scaled by the copy calibration), from the
+    committed profile summary; None when it holds no figure for the stage."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == stage:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def run_gpu(lib, geom, precision, warmup, steps):
@@ -206,7 +224,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(dom),
+            "traffic_source": "profiles/dambreak3d_8M_mixed.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)",
             "bytes_per_particle": dom_b,
             "avg_launch_ms": avg_ms,
         },
