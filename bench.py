@@ -125,6 +125,68 @@ def run_cpu_baseline(geom, threads):
     return {"value": sc.fluid.n * n / wall, "wall_s": wall, "acoustic_steps": n}
 
 
+def full_run(lib, geom, precision, t_end, observe_dt):
+    """The 3D dam break from rest to t_end: probe pressure, total mechanical
+    energy and surge front (UpperFrontInAxisDirectionCK along x) every
+    observe_dt of physical time; returns the series and the time to solution."""
+    from sphinxsys_amd import cases
+    sc = cases.DamBreak(lib, geom, precision)
+    sc.initialize()
+    sc.ctx.synchronize()
+    series = {"t": [], "p": [], "e": [], "front": []}
+
+    def obs():
+        p, e = sc.observe()
+        series["t"].append(sc.time)
+        series["p"].append(float(p[0]))
+        series["e"].append(e)
+        series["front"].append(sc.front.exec())
+
+    obs()
+    next_obs = observe_dt
+    t0 = time.perf_counter()
+    last_log = t0
+    while sc.time < t_end:
+        sc.advection_step()
+        if sc.time >= next_obs:
+            obs()
+            next_obs += observe_dt
+        now = time.perf_counter()
+        if now - last_log > 20.0:
+            log(f"  t = {sc.time:.4f} s after {sc.iterations} advection steps, {now - t0:.0f} s")
+            last_log = now
+    sc.ctx.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"series": series, "wall_s": wall, "advection_steps": sc.iterations,
+           "acoustic_steps": sc.acoustic_steps, "n_fluid": sc.fluid.n}
+    sc.close()
+    return out
+
+
+def compare_runs(a, b):
+    """Error of run a against run b (same observation times): probe pressure
+    history, its time average and first-impact peak, and the surge front."""
+    import numpy as np
+    pa, pb = np.array(a["p"]), np.array(b["p"])
+    fa, fb = np.array(a["front"]), np.array(b["front"])
+    n = min(len(pa), len(pb))
+    pa, pb, fa, fb = pa[:n], pb[:n], fa[:n], fb[:n]
+    t = np.array(b["t"][:n])
+    scale = max(np.abs(pb).max(), 1e-300)
+    ka, kb = int(np.argmax(pa)), int(np.argmax(pb))
+    return {
+        "probe_max_abs_err_over_peak": float(np.abs(pa - pb).max() / scale),
+        "probe_rel_l2": float(np.linalg.norm(pa - pb) / max(np.linalg.norm(pb), 1e-300)),
+        "probe_time_avg_rel_err": float(abs(pa.mean() - pb.mean()) / max(abs(pb.mean()), 1e-300)),
+        "probe_peak_rel_err": float(abs(pa[ka] - pb[kb]) / scale),
+        "probe_peak_time": [float(t[ka]), float(t[kb])],
+        "front_max_rel_err": float(np.abs(fa - fb).max() / max(np.abs(fb).max(), 1e-300)),
+        # first observation with the front within 0.2% of its largest value (the end wall)
+        "front_reaches_end_wall_at": [float(next((tt for tt, f in zip(t, fa) if f >= 0.998 * fb.max()), -1)),
+                                      float(next((tt for tt, f in zip(t, fb) if f >= 0.998 * fb.max()), -1))],
+    }
+
+
 def copy_calibration(nbytes=1 << 30, repeats=3):
     """Plain device-to-device copies of a known size through the HIP runtime:
     the reference reading of the HBM counters (FETCH_SIZE / WRITE_SIZE)."""
@@ -153,6 +215,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--full-run", type=float, default=0.0, metavar="T_END",
+                    help="instead of the throughput line: run the scene to T_END in FP32-mixed and FP64 "
+                         "and compare probe pressure and surge front (one JSON line)")
+    ap.add_argument("--observe", type=float, default=0.01, help="observation interval of --full-run")
+    ap.add_argument("--microbench", type=int, default=0, metavar="M",
+                    help="instead of the throughput line: the neighbour-search + density-summation "
+                         "microbenchmark on an M^3 jittered lattice (M=256: 16.8M particles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -169,6 +238,36 @@ def main():
     t0 = time.time()
     geom = cases.dambreak_3d(dp=args.dp)
     log(f"geometry: {len(geom.fluid_pos)} fluid, {len(geom.wall_pos)} wall in {time.time() - t0:.1f}s")
+
+    if args.microbench > 0:
+        import subprocess
+        exe = os.path.join(ROOT, "sphinxsys_amd", "lib", "microbench")
+        out = subprocess.run([exe, "--m", str(args.microbench), "--repeat", "5"], capture_output=True, text=True)
+        if out.returncode != 0:
+            log(out.stderr)
+            sys.exit(out.returncode)
+        res = [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")]
+        print(json.dumps({"metric": "neighbour search + density summation microbenchmark, stage ms",
+                          "lattice": f"{args.microbench}^3 jittered, mt19937_64 seed 20261202",
+                          "results": res}), flush=True)
+        return
+    if args.full_run > 0:
+        runs = {}
+        for name, prec in (("mixed", capi.MIXED), ("fp64", capi.FP64)):
+            log(f"{name} run to t = {args.full_run} s")
+            runs[name] = full_run(lib, geom, prec, args.full_run, args.observe)
+        cmp = compare_runs(runs["mixed"]["series"], runs["fp64"]["series"])
+        line = {"metric": "3D dam break time to solution and FP32-mixed error against FP64",
+                "t_end": args.full_run, "n_fluid": runs["mixed"]["n_fluid"],
+                "time_to_solution_s": {k: v["wall_s"] for k, v in runs.items()},
+                "particle_steps_per_s": {k: v["n_fluid"] * v["acoustic_steps"] / v["wall_s"]
+                                         for k, v in runs.items()},
+                "advection_steps": {k: v["advection_steps"] for k, v in runs.items()},
+                "acoustic_steps": {k: v["acoustic_steps"] for k, v in runs.items()},
+                "mixed_vs_fp64": cmp,
+                "series": {k: v["series"] for k, v in runs.items()}}
+        print(json.dumps(line), flush=True)
+        return
 
     log("FP32-mixed run")
     mixed = run_gpu(lib, geom, capi.MIXED, args.warmup, args.steps)

@@ -1,5 +1,6 @@
 """Build the HIP library sphinxsys_amd/lib/libsphx.so for gfx950 (in-tree) and the
-C++ dam-break case program sphinxsys_amd/lib/dambreak that drives it."""
+C++ programs that drive it: the dam-break case (sphinxsys_amd/lib/dambreak) and
+the neighbour-search microbenchmark (sphinxsys_amd/lib/microbench)."""
 import os
 import subprocess
 import sys
@@ -13,6 +14,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 CASE_EXE = os.path.join(LIB_DIR, "dambreak")
+MICROBENCH_EXE = os.path.join(LIB_DIR, "microbench")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           # packed FP32 instructions stay off: the compiler must not form v_pk_* FP32 ops
@@ -45,11 +47,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, *COMMON, "-shared", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
-    # the C++ case program on the CK mirror (host code only: it loads LIB at run time)
-    exe_tmp = CASE_EXE + ".tmp"
-    _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", exe_tmp, os.path.join(HERE, "host", "dambreak.cpp"), "-ldl"],
-         verbose)
-    os.replace(exe_tmp, CASE_EXE)
+    # the C++ programs on the CK mirror (host code only: they load LIB at run time)
+    for exe, src in ((CASE_EXE, "dambreak.cpp"), (MICROBENCH_EXE, "microbench.cpp")):
+        _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", exe + ".tmp", os.path.join(HERE, "host", src), "-ldl"],
+             verbose)
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

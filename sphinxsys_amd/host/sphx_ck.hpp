@@ -31,6 +31,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sphx
@@ -197,6 +198,15 @@ class SPHSystem
     const Api &api() const { return *api_; }
     sphx_context *handle() const { return h_; }
     void synchronize() const { api_->check(api_->sphx_context_synchronize(h_)); }
+    // HIP-event timing of the library's stages (sphx_profile_enable / _read)
+    void profile(bool enable) const { api_->check(api_->sphx_profile_enable(h_, enable ? 1 : 0)); }
+    std::pair<double, uint64_t> profile_read(const std::string &stage) const
+    {
+        double ms = 0.0;
+        uint64_t n = 0;
+        api_->check(api_->sphx_profile_read(h_, stage.c_str(), &ms, &n));
+        return {ms, n};
+    }
 
   private:
     static std::shared_ptr<Api> load_hip()
