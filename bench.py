@@ -219,6 +219,14 @@ def main():
                     help="instead of the throughput line: run the scene to T_END in FP32-mixed and FP64 "
                          "and compare probe pressure and surge front (one JSON line)")
     ap.add_argument("--observe", type=float, default=0.01, help="observation interval of --full-run")
+    ap.add_argument("--batched", type=int, default=0, metavar="N",
+                    help="instead of the throughput line: N independent 2D tanks (1k-64k particles), "
+                         "serial against batched on concurrent streams")
+    ap.add_argument("--served", type=int, default=0, metavar="N",
+                    help="instead of the throughput line: a Poisson stream of N 3D dam-break requests "
+                         "(20k-2M particles, t_end 1.0 s); p50/p99 time to solution")
+    ap.add_argument("--rate", type=float, default=0.5, help="arrival rate of --served (requests/s)")
+    ap.add_argument("--slots", type=int, default=16, help="concurrent scenes of --batched / --served")
     ap.add_argument("--microbench", type=int, default=0, metavar="M",
                     help="instead of the throughput line: the neighbour-search + density-summation "
                          "microbenchmark on an M^3 jittered lattice (M=256: 16.8M particles)")
@@ -239,6 +247,36 @@ def main():
     geom = cases.dambreak_3d(dp=args.dp)
     log(f"geometry: {len(geom.fluid_pos)} fluid, {len(geom.wall_pos)} wall in {time.time() - t0:.1f}s")
 
+    if args.batched > 0 or args.served > 0:
+        from sphinxsys_amd import serving
+        if args.batched > 0:
+            specs = serving.batched_2d_specs(args.batched)
+            what = f"{args.batched} independent 2D dam-break tanks, 40 advection steps each, FP32-mixed"
+        else:
+            specs = serving.served_3d_specs(args.served, rate=args.rate)
+            what = (f"Poisson stream of {args.served} 3D dam breaks at {args.rate}/s, "
+                    "20k-2M fluid particles, t_end 1.0 s, FP32-mixed")
+        t0 = time.time()
+        for sp in specs:
+            sp.geometry()  # the requests carry their particle sets
+        log(f"{len(specs)} scenes generated in {time.time() - t0:.1f}s "
+            f"({sum(sp.fluid_count() for sp in specs)} fluid particles)")
+        log("serial (one scene at a time, back to back)")
+        res1, wall1 = serving.Server(lib, max_active=1).run(specs, realtime=False)
+        serial = serving.summarize(res1, wall1)
+        log(f"  {serial['particle_steps_per_s']:.3g} particle-steps/s in {wall1:.1f}s")
+        log(f"concurrent ({args.slots} slots{', arrivals in real time' if args.served else ''})")
+        res2, wall2 = serving.Server(lib, max_active=args.slots).run(specs, realtime=args.served > 0)
+        conc = serving.summarize(res2, wall2)
+        log(f"  {conc['particle_steps_per_s']:.3g} particle-steps/s in {wall2:.1f}s, "
+            f"p50 {conc['latency_p50_s']:.2f}s p99 {conc['latency_p99_s']:.2f}s")
+        same = all(a.acoustic_steps == b.acoustic_steps and a.physical_time == b.physical_time
+                   for a, b in zip(res1, res2))
+        print(json.dumps({"metric": "aggregate particle-steps/s and time to solution of independent scenes",
+                          "workload": what, "serial": serial, "concurrent": conc,
+                          "speedup": conc["particle_steps_per_s"] / serial["particle_steps_per_s"],
+                          "same_steps_and_time_alone_and_in_mix": same}), flush=True)
+        return
     if args.microbench > 0:
         import subprocess
         exe = os.path.join(ROOT, "sphinxsys_amd", "lib", "microbench")
