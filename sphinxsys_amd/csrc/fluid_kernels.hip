@@ -27,18 +27,24 @@ constexpr int kMaxWalls = 4;
 static_assert(kThreads == (int)kStageBlock, "staging blocks are pass workgroups");
 inline unsigned grid_for(size_t n, size_t per) { return (unsigned)((n + per - 1) / per); }
 
+// 1 / |r| of a pair; a coincident pair (r2 = 0) gets a finite value, so that
+// its unit vector r * inv_sqrt(r2) is zero as Eigen's normalized() of a zero
+// vector, and its distance r2 * inv_sqrt(r2) is zero
 template <class T>
 __device__ __forceinline__ T inv_sqrt(T r2);
 template <>
 __device__ __forceinline__ float inv_sqrt<float>(float r2)
 {
-    return r2 > 0.0f ? rsqrtf(r2) : 0.0f;
+    return rsqrtf(fmaxf(r2, 1e-30f));
 }
 template <>
 __device__ __forceinline__ double inv_sqrt<double>(double r2)
 {
-    return r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+    return 1.0 / sqrt(fmax(r2, 1e-300));
 }
+
+__device__ __forceinline__ float frac_part(float x) { return __builtin_amdgcn_fractf(x); }
+__device__ __forceinline__ double frac_part(double x) { return __builtin_amdgcn_fract(x); }
 
 template <class T>
 struct WallArgs
@@ -117,14 +123,16 @@ __device__ __forceinline__ void stage_poly(PolyLDS<T> &lds, const KernelPoly<T> 
         lds.c[k >> 2][k & 3] = grad ? g->dw[k >> 2][k & 3] : g->w[k >> 2][k & 3];
 }
 
+// kernel value or gradient at distance r (x_per_r = inv_h / dq of the relation);
+// neighbours have q < 2, so loc stays within the table
 template <class T>
-__device__ __forceinline__ T eval_poly(const PolyLDS<T> &lds, T q, T inv_dq, T dq)
+__device__ __forceinline__ T eval_poly(const PolyLDS<T> &lds, T r, T x_per_r)
 {
-    int loc = (int)(q * inv_dq);
-    loc = loc < 0 ? 0 : (loc > kPolyIntervals - 1 ? kPolyIntervals - 1 : loc);
-    T t = q - T(loc) * dq;
+    const T x = r * x_per_r;
+    const int loc = min((int)x, kPolyIntervals - 1);
+    const T f = frac_part(x);
     const T *c = lds.c[loc];
-    return ((c[3] * t + c[2]) * t + c[1]) * t + c[0];
+    return ((c[3] * f + c[2]) * f + c[1]) * f + c[0];
 }
 
 // walk the neighbour entries of one particle (slice layout: entry k at
@@ -208,7 +216,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and p of the block's distinct neighbours
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
-    T *s_p = (T *)(s_pv + a.stage_cap);
+    T *s_p = (T *)(s_pv + kStageCap);
     const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
     const bool staged = n_staged != kStageOverflow;
     if (staged)
@@ -225,7 +233,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     if (i >= a.n)
         return;
     const T dt = T(dt_d);
-    const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
+    const T x_per_r = a.poly->x_per_r;
     const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
     const T p_i = a.p[i];
     // inner: force_sum -= 2 AverageP dW V_j e_ij, AverageP = (p_i + p_j)/2 for one fluid
@@ -234,8 +242,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
         const T ir = inv_sqrt(r2);
-        const T q = r2 * ir * inv_h;
-        const T dWV = eval_poly(poly_in, q, inv_dq, dq) * pj.vol;
+        const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
         const T s = (p_i + p_j) * dWV * ir;
         fx += s * dx;
         fy += s * dy;
@@ -268,7 +275,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
         for (int k = 0; k < a.n_wall; ++k)
         {
             const WallArgs<T> &w = a.wall[k];
-            const T w_inv_h = w.poly->inv_h;
+            const T w_x_per_r = w.poly->x_per_r;
             T wx = 0, wy = 0, wz = 0, wcd = 0;
             const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
             for (uint32_t k2 = 0; k2 < mc; ++k2)
@@ -279,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
                 const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
                 const T ir = inv_sqrt(r2);
                 const T r = r2 * ir;
-                const T dWV = eval_poly(poly_w[k], r * w_inv_h, inv_dq, dq) * pj.vol;
+                const T dWV = eval_poly(poly_w[k], r, w_x_per_r) * pj.vol;
                 const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
                 T face = -((apx - w.acc[0][j]) * ex + (apy - w.acc[1][j]) * ey);
                 if (D == 3)
@@ -323,12 +330,13 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     }
 }
 
-// limiter of AcousticRiemannSolverCK: TruncatedLinear(3) on inv_c0 * max(u, 0)
+// DissipativePJump of the Riemann solvers divided by rho0c0_geo_ave (the
+// factor is applied once per particle): u times the limiter of
+// AcousticRiemannSolverCK, TruncatedLinear(3) on inv_c0 * max(u, 0), or 1
 template <class T>
-__device__ __forceinline__ T p_jump(const FluidArgs<T> &a, T u)
+__device__ __forceinline__ T p_jump_scaled(const FluidArgs<T> &a, T u)
 {
-    T lim = a.riemann == SPHX_RIEMANN_ACOUSTIC ? dmin(T(3) * (a.inv_c0_ave * dmax(u, T(0))), T(1)) : T(1);
-    return a.rho0c0_geo_ave * u * lim;
+    return a.riemann == SPHX_RIEMANN_ACOUSTIC ? u * dmin(T(3) * (a.inv_c0_ave * dmax(u, T(0))), T(1)) : u;
 }
 
 //---------------------------------------------------------------------------
@@ -352,7 +360,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and velocity
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
-    Rec4<T> *s_v = s_pv + a.stage_cap;
+    Rec4<T> *s_v = s_pv + kStageCap;
     const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
     const bool staged = n_staged != kStageOverflow;
     if (staged)
@@ -370,7 +378,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
     if (i < a.n)
     {
         const T dt = T(dt_d);
-        const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
+        const T x_per_r = a.poly->x_per_r;
         const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
         const Rec4<T> vi = a.vel[i];
         // divergence: 2 (v_i - v_ave).e dW V = (v_i - v_j).e dW V for one fluid
@@ -379,8 +387,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
             const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
             const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
             const T ir = inv_sqrt(r2);
-            const T q = r2 * ir * inv_h;
-            const T dWV = eval_poly(poly_in, q, inv_dq, dq) * pj.vol;
+            const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
             const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
             T u = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey;
             if (D == 3)
@@ -388,7 +395,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
             div += u * dWV;
             if (RIEMANN)
             {
-                const T s = p_jump(a, u) * dWV;
+                const T s = p_jump_scaled(a, u) * dWV;
                 px += s * ex;
                 py += s * ey;
                 if (D == 3)
@@ -403,11 +410,12 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         const T C_i = full_C<P>(a.C[i]);
         const T vol_i = pi.vol;
         T dC = a.dC[i] + div * C_i;
-        T Fx = px * vol_i, Fy = py * vol_i, Fz = D == 3 ? pz * vol_i : T(0);
+        const T zv = a.rho0c0_geo_ave * vol_i;
+        T Fx = px * zv, Fy = py * zv, Fz = D == 3 ? pz * zv : T(0);
         for (int k = 0; k < a.n_wall; ++k)
         {
             const WallArgs<T> &w = a.wall[k];
-            const T w_inv_h = w.poly->inv_h;
+            const T w_x_per_r = w.poly->x_per_r;
             T wdiv = 0, wx = 0, wy = 0, wz = 0;
             const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
             for (uint32_t k2 = 0; k2 < mc; ++k2)
@@ -417,7 +425,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
                 const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
                 const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
                 const T ir = inv_sqrt(r2);
-                const T dWV = eval_poly(poly_w[k], r2 * ir * w_inv_h, inv_dq, dq) * pj.vol;
+                const T dWV = eval_poly(poly_w[k], r2 * ir, w_x_per_r) * pj.vol;
                 const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
                 const T vdx = T(2) * (vi.x - w.vel[0][j]), vdy = T(2) * (vi.y - w.vel[1][j]);
                 const T vdz = D == 3 ? T(2) * (vi.z - w.vel[2][j]) : T(0);
@@ -436,7 +444,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
                     T u = vdx * fnx + vdy * fny;
                     if (D == 3)
                         u += vdz * fnz;
-                    const T s = p_jump(a, u) * dWV;
+                    const T s = p_jump_scaled(a, u) * dWV;
                     wx += s * fnx;
                     wy += s * fny;
                     if (D == 3)
@@ -444,10 +452,10 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
                 }
             }
             dC += wdiv * C_i;
-            Fx += wx * vol_i;
-            Fy += wy * vol_i;
+            Fx += wx * zv;
+            Fy += wy * zv;
             if (D == 3)
-                Fz += wz * vol_i;
+                Fz += wz * zv;
         }
         a.dC[i] = dC;
         a.force[0][i] = Fx;
@@ -522,15 +530,15 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
     __syncthreads();
     if (i >= a.n)
         return;
-    const T inv_h = a.poly->inv_h, inv_dq = a.poly->inv_dq, dq = a.poly->dq;
+    const T x_per_r = a.poly->x_per_r;
     const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
     T s = a.poly->w0 * a.Vol_ref[i];
     // W_ij Vref_j (pj.vol carries Vref_j)
     auto pair = [&](const PV<T, D> &pj) {
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
-        const T q = sqrt(r2) * inv_h;
-        s += eval_poly(poly_in, q, inv_dq, dq) * pj.vol;
+        const T r = sqrt(r2);
+        s += eval_poly(poly_in, r, x_per_r) * pj.vol;
     };
     const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
     if (staged)
@@ -544,7 +552,7 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
     for (int k = 0; k < a.n_wall; ++k)
     {
         const WallArgs<T> &w = a.wall[k];
-        const T w_inv_h = w.poly->inv_h;
+        const T w_x_per_r = w.poly->x_per_r;
         const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
         for (uint32_t k2 = 0; k2 < mc; ++k2)
         {
@@ -552,7 +560,8 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
             const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
             const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
             const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
-            s += eval_poly(poly_w[k], sqrt(r2) * w_inv_h, inv_dq, dq) * w.Vol_ref[j];
+            const T r = sqrt(r2);
+            s += eval_poly(poly_w[k], r, w_x_per_r) * w.Vol_ref[j];
         }
     }
     a.S[i] = s;
@@ -818,12 +827,11 @@ __global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> 
         const PV<T, D> pj = load_pv<T, D>(tar, j);
         const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
-        const T qq = sqrt(r2) * poly->inv_h;
-        int loc = (int)(qq * poly->inv_dq);
-        loc = loc < 0 ? 0 : (loc > kPolyIntervals - 1 ? kPolyIntervals - 1 : loc);
-        const T t = qq - T(loc) * poly->dq;
+        const T x = sqrt(r2) * poly->x_per_r;
+        const int loc = min((int)x, kPolyIntervals - 1);
+        const T f = frac_part(x);
         const T *c = poly->w[loc];
-        const T W = ((c[3] * t + c[2]) * t + c[1]) * t + c[0];
+        const T W = ((c[3] * f + c[2]) * f + c[1]) * f + c[0];
         const T weight = W * pj.vol;
         q += weight * data[j];
         w += weight;
@@ -1284,11 +1292,14 @@ static void expand_table(const double *data, long double factor, T (*out)[4], do
         cubic(-d, d, 2 * d, b1);
         cubic(-d, 0.0L, 2 * d, b2);
         cubic(-d, 0.0L, d, b3);
+        // coefficient of t^k times dq^k: the polynomial in f = t / dq
+        long double dqk = 1.0L;
         for (int k = 0; k < 4; ++k)
         {
             long double v = b0[k] / dl0 * data[i - 1] + b1[k] / dl1 * data[i] + b2[k] / dl2 * data[i + 1] +
                             b3[k] / dl3 * data[i + 2];
-            out[loc][k] = T(v * factor);
+            out[loc][k] = T(v * dqk * factor);
+            dqk *= d;
         }
     }
 }
@@ -1324,8 +1335,7 @@ static sphx_status build_poly_t(sphx_relation *rel)
     expand_table(w, (long double)fW, kp.w, dq);
     expand_table(dw, (long double)fdW, kp.dw, dq);
     kp.inv_h = T(inv_h);
-    kp.dq = T(dq);
-    kp.inv_dq = T(1.0 / dq);
+    kp.x_per_r = T(inv_h * (1.0 / dq));
     // W0 = src_inv_h^D dimfac W(0); interpolateCubic at q = 0 returns data[1] scaled by the basis
     const double f1 = 0.0, f0 = dq, f2 = -dq, f3 = -2 * dq;
     const double dl0 = (-1.0 * dq) * (-2.0 * dq) * (-3.0 * dq), dl1 = dq * (-1.0 * dq) * (-2.0 * dq);

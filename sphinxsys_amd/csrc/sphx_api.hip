@@ -697,7 +697,7 @@ static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_lin
         // records per distinct neighbour; keep 8 KiB for the static tables
         // (MIXED only: FP64 records would halve the occupancy of the passes)
         const size_t rec2 = 2 * (src->P == P_FP64 ? sizeof(Rec4<double>) : sizeof(Rec4<float>));
-        const size_t cap = 1536;
+        const size_t cap = kStageCap;
         if (src->P == P_MIXED && cap * rec2 + 8192 <= std::min<size_t>(src->ctx->max_lds, 65536))
         {
             const size_t nblocks = (src->n + kStageBlock - 1) / kStageBlock;

@@ -49,17 +49,18 @@ struct Prec<P_MIXED>
     static constexpr bool c_minus_one = true;
 };
 
-// Tabulated Wendland C2 kernel re-expressed per interval as a cubic in
-// t = q - loc*dq (the Lagrange form of kernel_tabulated_ck.h:45-58 expanded on
-// the host), premultiplied by the dimension and smoothing-length factors.
+// Tabulated Wendland C2 kernel re-expressed per interval as a cubic in the
+// fraction f = x - loc of the table coordinate x = q / dq = r inv_h / dq (the
+// Lagrange form of kernel_tabulated_ck.h:45-58 expanded on the host),
+// premultiplied by the dimension and smoothing-length factors.
 constexpr int kPolyIntervals = 21; // loc in [0, 20]; q in [0, 2.1)
 
 template <class T>
 struct KernelPoly
 {
-    T w[kPolyIntervals][4];  // W  = sum_k w[loc][k]  t^k   (factor inv_h^D dimfac folded in)
-    T dw[kPolyIntervals][4]; // dW = sum_k dw[loc][k] t^k   (factor inv_h^(D+1) dimfac folded in)
-    T inv_h, dq, inv_dq, w0; // w0 = W0 * (src) with its own factor
+    T w[kPolyIntervals][4];  // W  = sum_k w[loc][k]  f^k   (factor inv_h^D dimfac folded in)
+    T dw[kPolyIntervals][4]; // dW = sum_k dw[loc][k] f^k   (factor inv_h^(D+1) dimfac folded in)
+    T inv_h, x_per_r, w0;    // x_per_r = inv_h / dq; w0 = W0 of the source with its own factor
 };
 
 // Mesh of a cell linked list, base_mesh.cpp:6-16 / base_mesh.hxx:9-26.
