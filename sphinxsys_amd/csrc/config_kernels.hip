@@ -328,7 +328,54 @@ struct RelationArgs
     uint32_t *neighbor_size;
     const uint32_t *chunk_offset;
     uint32_t *slice;
+    // per target cell: 1 if a target particle lies within the cell's search
+    // box (null: not computed); sources in other cells have no neighbours
+    const uint8_t *near;
 };
+
+// near[c] of a cell linked list: any particle in the +-1 cell search box of c
+template <int D>
+__global__ void k_near_flags(const uint32_t *cell_offset, MeshParams m, uint8_t *near, uint32_t ncells)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells)
+        return;
+    int cc[3];
+    if (D == 3)
+    {
+        cc[2] = (int)(c % (uint32_t)m.all_cells[2]);
+        cc[1] = (int)((c / (uint32_t)m.all_cells[2]) % (uint32_t)m.all_cells[1]);
+        cc[0] = (int)(c / ((uint32_t)m.all_cells[2] * (uint32_t)m.all_cells[1]));
+    }
+    else
+    {
+        cc[1] = (int)(c % (uint32_t)m.all_cells[1]);
+        cc[0] = (int)(c / (uint32_t)m.all_cells[1]);
+    }
+    bool any = false;
+    for (int dl = -1; dl <= 1; ++dl)
+        for (int dm = -1; dm <= (D == 3 ? 1 : -1); ++dm)
+        {
+            const int l = cc[0] + dl;
+            if (l < 0 || l >= m.all_cells[0])
+                continue;
+            if (D == 3)
+            {
+                const int mm = cc[1] + dm;
+                if (mm < 0 || mm >= m.all_cells[1])
+                    continue;
+                const int lo2 = max(0, cc[2] - 1), hi2 = min(m.all_cells[2], cc[2] + 2);
+                const uint32_t first = (uint32_t)(l * m.all_cells[1] * m.all_cells[2] + mm * m.all_cells[2] + lo2);
+                any |= cell_offset[first + (uint32_t)(hi2 - lo2)] > cell_offset[first];
+            }
+            else
+            {
+                const int lo1 = max(0, cc[1] - 1), hi1 = min(m.all_cells[1], cc[1] + 2);
+                any |= cell_offset[(uint32_t)(l * m.all_cells[1] + hi1)] > cell_offset[(uint32_t)(l * m.all_cells[1] + lo1)];
+            }
+        }
+    near[c] = any ? 1 : 0;
+}
 
 // slices -> CSR NeighborIndex (the reference's layout), for reads
 __global__ void k_slice_to_csr(const uint32_t *count, const uint32_t *chunk, const uint32_t *slice,
@@ -639,6 +686,10 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
         xf = a.src_f[i];
         for (int k2 = 0; k2 < D; ++k2)
             exact_lane |= !(fabs(xi[k2]) <= a.pre_xmax);
+    }
+    // sources whose search box holds no target particle keep empty rows
+    if (valid && (!a.near || a.near[linear_cell<D>(a.m, c)]))
+    {
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
@@ -1119,6 +1170,27 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
     if ((s = radix_sort_pairs(ctx, cll->keys, cll->particle_index, b->n, cll->key_bits)))
         return s;
     cll->built = true;
+    ++cll->build_count;
+    return SPHX_OK;
+}
+
+// near flags of a (static) target cell linked list, recomputed after each rebuild
+sphx_status cell_near_flags(sphx_cell_linked_list *cll)
+{
+    sphx_context *ctx = cll->body->ctx;
+    if (cll->near && cll->near_build == cll->build_count)
+        return SPHX_OK;
+    if (!cll->near)
+        SPHX_HIP(hipMalloc(&cll->near, (size_t)cll->ncells));
+    const uint32_t nc = (uint32_t)cll->ncells;
+    if (cll->body->D == 2)
+        hipLaunchKernelGGL(k_near_flags<2>, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,
+                           cll->mesh, cll->near, nc);
+    else
+        hipLaunchKernelGGL(k_near_flags<3>, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,
+                           cll->mesh, cll->near, nc);
+    SPHX_HIP(hipGetLastError());
+    cll->near_build = cll->build_count;
     return SPHX_OK;
 }
 
@@ -1136,6 +1208,7 @@ static RelationArgs relation_args(sphx_relation *rel)
     a.neighbor_size = rel->neighbor_size;
     a.chunk_offset = rel->chunk_offset;
     a.slice = rel->slice;
+    a.near = rel->inner ? nullptr : rel->cll->near;
     // pre-filter bounds: float rounding of positions bounded by |x| <= xmax
     // (target mesh extent + 2.1 h) gives a relative error of the squared
     // distance below 2.2e-7 xmax/h + 5.6e-7 near the cut-off; the margin is 4x that

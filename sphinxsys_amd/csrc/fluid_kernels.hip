@@ -145,10 +145,12 @@ __device__ __forceinline__ void for_each_neighbor(const I *entries, uint32_t bas
     uint32_t k = 0;
     for (; k + kBatch <= count; k += kBatch)
     {
+        // one pointer per batch: the 8 loads take immediate offsets (64 entries apart)
+        const I *row = entries + base + ((size_t)k << 6);
         uint32_t idx[kBatch];
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; ++q)
-            idx[q] = entries[base + ((k + q) << 6)];
+            idx[q] = row[q << 6];
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; ++q)
             f(idx[q]);

@@ -612,6 +612,8 @@ sphx_status sphx_cell_linked_list_destroy(sphx_cell_linked_list *c)
     (void)hipFree(c->cell_offset);
     (void)hipFree(c->particle_index);
     (void)hipFree(c->keys);
+    if (c->near)
+        (void)hipFree(c->near);
     delete c;
     return SPHX_OK;
 }
@@ -771,6 +773,8 @@ sphx_status sphx_update_relation(sphx_relation *r)
     const size_t n = r->source->n;
     const size_t nchunks = (n + 63) / 64;
     uint32_t kmax = r->kmax ? r->kmax : (r->source->D == 3 ? 96u : 32u);
+    if (!r->inner && (s = cell_near_flags(c)))
+        return s;
     bool scan = false, done = false;
     for (int attempt = 0; attempt < 5 && !done; ++attempt)
     {

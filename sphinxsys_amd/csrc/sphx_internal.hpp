@@ -104,6 +104,9 @@ struct sphx_cell_linked_list
     uint32_t *keys = nullptr;           // n (sort keys scratch)
     bool built = false;
     uint64_t built_version = 0;         // body config_version the list was built for
+    uint64_t build_count = 0;           // updates so far
+    uint8_t *near = nullptr;            // per cell: a particle within the +-1 search box (contact targets)
+    uint64_t near_build = 0;            // build_count the flags were computed for
 };
 
 struct sphx_relation
@@ -150,6 +153,8 @@ sphx_status relation_to_csr(sphx_relation *rel);
 sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed, bool scan);
 // the LDS staging sets of a staged relation, from its finished slices
 sphx_status relation_stage(sphx_relation *rel);
+// per-cell flags of a target cell list: any particle within the search box
+sphx_status cell_near_flags(sphx_cell_linked_list *cll);
 sphx_status morton_sequence(sphx_body *b, sphx_cell_linked_list *cll);
 sphx_status gather_u32(sphx_context *ctx, uint32_t *field, const uint32_t *perm, size_t n);
 sphx_status gather_real(sphx_body *b, void **field, const uint32_t *perm);
