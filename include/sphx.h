@@ -70,7 +70,12 @@ enum {
     SPHX_FIELD_SORTED_ID = 17,        /* "SortedID"              uint32 */
     SPHX_FIELD_SORT_SEQUENCE = 18,    /* sort keys after the last sort, uint32 */
     SPHX_FIELD_INTERPOLATED = 19,     /* observer result of sphx_interpolate_contact */
-    SPHX_FIELD_COUNT = 20
+    SPHX_FIELD_CORRECTION_MATRIX = 20,/* "LinearCorrectionMatrix" Matd: dim*dim doubles per
+                                         particle, row-major; initial value identity */
+    SPHX_FIELD_INDICATOR = 21,        /* "Indicator"               int32 (1: free surface) */
+    SPHX_FIELD_PREVIOUS_INDICATOR = 22,/* "PreviousSurfaceIndicator" int32, initial 1   */
+    SPHX_FIELD_POSITION_DIVERGENCE = 23,/* "PositionDivergence"    Real  */
+    SPHX_FIELD_COUNT = 24
 };
 
 /* Riemann solvers (reference: riemann_solver_ck.h:166-172,
@@ -83,6 +88,10 @@ enum {
 
 /* Density regularization flow types (reference: density_regularization.h:146-184). */
 enum { SPHX_FLOW_INTERNAL = 0, SPHX_FLOW_FREE_SURFACE = 1 };
+
+/* Kernel corrections of the acoustic half steps (reference: NoKernelCorrectionCK,
+ * LinearCorrectionCK, general_dynamics/kernel_correction_ck.h:153-187). */
+enum { SPHX_CORRECTION_NONE = 0, SPHX_CORRECTION_LINEAR = 1 };
 
 typedef struct sphx_context sphx_context;
 typedef struct sphx_body sphx_body;
@@ -194,6 +203,32 @@ sphx_status sphx_acoustic_step_1st_half(sphx_relation *inner, sphx_relation *con
 /* reference: ... AcousticStep2ndHalf<...> (fluid_dynamics/acoustic_step_2nd_half.hpp:33-137). */
 sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *const *walls,
                                         int n_wall, int riemann, double dt);
+/* reference: InteractionDynamicsCK<P, LinearCorrectionMatrixComplex> built with
+ * DynamicsArgs(inner, alpha) and the wall contact (general_dynamics/kernel_correction_ck.hpp:37-90,
+ * test_3d_dambreak_sycl/dambreak.cpp:123-124): B_i = -sum_j r_ij (x) grad W_ij V_j over the
+ * inner and contact neighbours, then B_i = w inverseTikhonov(B_i, 1e-8) + (1 - w) I with
+ * w = det / (det + max(alpha - det, 0)).  Result in SPHX_FIELD_CORRECTION_MATRIX. */
+sphx_status sphx_linear_correction_matrix(sphx_relation *inner, sphx_relation *const *contact,
+                                          int n_contact, double alpha);
+/* reference: AcousticStep1stHalfWithWallRiemannCorrectionCK and its Riemann variants,
+ * AcousticStep1stHalf<Inner<OneLevel,R,LinearCorrectionCK>, Contact<Wall,R,LinearCorrectionCK>>
+ * (acoustic_step_1st_half.h:199-201, .hpp:89-111, 157-180): the pressure force with
+ * AverageP(B_j p_i, B_i p_j) inside the fluid and (p_i + p_w) B_i at the wall, B from the
+ * last sphx_linear_correction_matrix (or a SPHX_FIELD_CORRECTION_MATRIX write). */
+sphx_status sphx_acoustic_step_1st_half_corrected(sphx_relation *inner, sphx_relation *const *walls,
+                                                  int n_wall, int riemann, double dt);
+/* reference: AcousticStep2ndHalfWithWallRiemannCorrectionCK (acoustic_step_2nd_half.h:189-191,
+ * .hpp:53-73, 117-137): velocity divergence with B_i e_ij. */
+sphx_status sphx_acoustic_step_2nd_half_corrected(sphx_relation *inner, sphx_relation *const *walls,
+                                                  int n_wall, int riemann, double dt);
+/* reference: InteractionDynamicsCK<P, FreeSurfaceIndicationComplexSpatialTemporalCK>
+ * (general_dynamics/surface_indication/surface_indication_ck.hpp:48-178): position divergence
+ * -sum dW_ij V_j r_ij (inner, held above threshold 0.75 dim for a particle that was not and is
+ * not next to the previous free surface, then the contact part), Indicator = 1 unless the
+ * divergence exceeds the threshold with no surface-side neighbour closer than h;
+ * PreviousSurfaceIndicator = Indicator. */
+sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *const *contact,
+                                         int n_contact);
 
 /* ---- reduce dynamics -------------------------------------------------------- */
 /* reference: ReduceDynamicsCK<P, AcousticTimeStepCK<WeaklyCompressibleFluid>>
@@ -221,6 +256,7 @@ sphx_status sphx_interpolate_contact(sphx_relation *observer_contact, int field,
  * One advection step of the weakly-compressible loop of the CK dam-break
  * cases (tests/tests_sycl/3d_examples/test_3d_dambreak_sycl/dambreak.cpp:188-222):
  * compression summation -> regularization -> setup -> advection dt ->
+ * [free-surface indication] -> [linear correction matrix] ->
  * acoustic sub-steps (dt, 1st half, 2nd half) until their sum reaches the
  * advection dt -> position update -> [sort when sort_now] -> cell linked
  * list -> relations.  The acoustic time step size stays on the device.
@@ -234,6 +270,10 @@ typedef struct sphx_advection_config {
     int riemann;           /* SPHX_RIEMANN_*                              */
     int flow_type;         /* SPHX_FLOW_*                                 */
     int sort_now;          /* run ParticleSortCK after the position update */
+    int correction;        /* SPHX_CORRECTION_*: LINEAR runs LinearCorrectionMatrixComplex
+                              before the sub-steps and the corrected half steps         */
+    double correction_alpha; /* LinearCorrectionMatrix alpha (0.5 in the CK cases)    */
+    int surface_indication; /* run FreeSurfaceIndicationComplexSpatialTemporalCK      */
 } sphx_advection_config;
 
 sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *walls, int n_wall,

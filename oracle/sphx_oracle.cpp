@@ -62,8 +62,10 @@ struct Body : BodyBase
     R h;
     std::unique_ptr<TabulatedKernel<R>> kernel;
     std::vector<V> pos, vel, force, force_prior, dpos, normal, wall_vel, wall_acc;
-    std::vector<R> Vol, mass, rho, p, C, dC, Vol_ref, S, interp;
+    std::vector<R> Vol, mass, rho, p, C, dC, Vol_ref, S, interp, pos_div;
     std::vector<UInt> original_id, sorted_id, sequence;
+    std::vector<Mat<R, D>> correction;         // "LinearCorrectionMatrix", initial identity
+    std::vector<int32_t> indicator, prev_indicator; // "Indicator" (0), "PreviousSurfaceIndicator" (1)
 
     explicit Body(const sphx_body_desc &d)
     {
@@ -78,8 +80,11 @@ struct Body : BodyBase
         dpos.assign(n, z), normal.assign(n, z), wall_vel.assign(n, z), wall_acc.assign(n, z);
         Vol.assign(n, R(0)), mass.assign(n, R(0)), rho.assign(n, R(d.rho0)), p.assign(n, R(0));
         C.assign(n, R(1)), dC.assign(n, R(0)), Vol_ref.assign(n, R(0)), S.assign(n, R(1));
-        interp.assign(n, R(0));
+        interp.assign(n, R(0)), pos_div.assign(n, R(0));
         original_id.resize(n), sorted_id.resize(n), sequence.assign(n, 0);
+        // kernel_correction_ck.hpp:12-15; surface_indication_ck.hpp:17-18, 41-44
+        correction.assign(n, Mat<R, D>::Identity());
+        indicator.assign(n, 0), prev_indicator.assign(n, 1);
         for (UInt i = 0; i < n; ++i)
             original_id[i] = sorted_id[i] = i;
     }
@@ -256,6 +261,12 @@ struct RiemannParams
     }
     // AverageP / AverageV, riemann_solver_ck.hpp:17-30
     R AverageP(R p_i, R p_j) const { return imp.inv_rho0c0_sum * (p_i * imp.rho0c0 + p_j * imp.rho0c0); }
+    // the same with LinearCorrectionCK's matrix arguments (CorrectionDataType = Matd)
+    template <int D>
+    Mat<R, D> AverageP(const Mat<R, D> &p_i, const Mat<R, D> &p_j) const
+    {
+        return imp.inv_rho0c0_sum * (p_i * imp.rho0c0 + p_j * imp.rho0c0);
+    }
     template <int D>
     Vec<R, D> AverageV(const Vec<R, D> &vi, const Vec<R, D> &vj) const
     {
@@ -263,9 +274,10 @@ struct RiemannParams
     }
 };
 
+// corrected: LinearCorrectionCK (correction_(i) = B_i), else NoKernelCorrectionCK (1.0)
 template <class R, int D>
 static sphx_status acoustic_1st(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
-                                int riemann, double dt_d)
+                                int riemann, double dt_d, bool corrected)
 {
     Body<R, D> &b = as<R, D>(inner->source);
     const R dt = R(dt_d);
@@ -290,7 +302,11 @@ static sphx_status acoustic_1st(sphx_relation *inner, sphx_relation *const *wall
             Vec<R, D> r = b.pos[i] - b.pos[j];
             R dW_ijV_j = nk.dW(r) * b.Vol[j];
             Vec<R, D> e_ij = normalized(r);
-            force_sum -= rs.AverageP(b.p[i], b.p[j]) * R(2.0) * dW_ijV_j * e_ij;
+            if (corrected)
+                force_sum -= rs.AverageP(b.correction[j] * b.p[i], b.correction[i] * b.p[j]) * R(2.0) *
+                             dW_ijV_j * e_ij;
+            else
+                force_sum -= rs.AverageP(b.p[i], b.p[j]) * R(2.0) * dW_ijV_j * e_ij;
             compression_dissipation += rs.DissipativeUJump(b.p[i] - b.p[j]) * dW_ijV_j;
         }
         b.force[i] += force_sum * b.Vol[i];
@@ -315,7 +331,10 @@ static sphx_status acoustic_1st(sphx_relation *inner, sphx_relation *const *wall
                 Vec<R, D> neg_e = e_ij * R(-1);
                 R face_wall_external_acceleration = dot(b.force_prior[i] / b.mass[i] - wb.wall_acc[j], neg_e);
                 R p_j_in_wall = b.p[i] + b.rho[i] * r_ij * std::max(R(0), face_wall_external_acceleration);
-                force_sum -= (b.p[i] + p_j_in_wall) * dW_ijV_j * e_ij;
+                if (corrected)
+                    force_sum -= (b.p[i] + p_j_in_wall) * b.correction[i] * dW_ijV_j * e_ij;
+                else
+                    force_sum -= (b.p[i] + p_j_in_wall) * dW_ijV_j * e_ij;
                 compression_dissipation += rs.DissipativeUJump(b.p[i] - p_j_in_wall) * dW_ijV_j;
             }
             b.force[i] += force_sum * b.Vol[i];
@@ -331,7 +350,7 @@ static sphx_status acoustic_1st(sphx_relation *inner, sphx_relation *const *wall
 
 template <class R, int D>
 static sphx_status acoustic_2nd(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
-                                int riemann, double dt_d)
+                                int riemann, double dt_d, bool corrected)
 {
     Body<R, D> &b = as<R, D>(inner->source);
     const R dt = R(dt_d);
@@ -351,7 +370,8 @@ static sphx_status acoustic_2nd(sphx_relation *inner, sphx_relation *const *wall
             R dW_ijV_j = nk.dW(r) * b.Vol[j];
             Vec<R, D> e_ij = normalized(r);
             Vec<R, D> vel_ave = rs.AverageV(b.vel[i], b.vel[j]);
-            divergence_sum += R(2.0) * dot(b.vel[i] - vel_ave, e_ij) * dW_ijV_j;
+            const Vec<R, D> Be = corrected ? b.correction[i] * e_ij : e_ij;
+            divergence_sum += R(2.0) * dot(b.vel[i] - vel_ave, Be) * dW_ijV_j;
             R u_jump = dot(b.vel[i] - b.vel[j], e_ij);
             p_dissipation += rs.DissipativePJump(u_jump) * dW_ijV_j * e_ij;
         }
@@ -374,7 +394,8 @@ static sphx_status acoustic_2nd(sphx_relation *inner, sphx_relation *const *wall
                 R dW_ijV_j = wk.dW(r) * wb.Vol[j];
                 Vec<R, D> e_ij = normalized(r);
                 Vec<R, D> vel_diff = R(2.0) * (b.vel[i] - wb.wall_vel[j]);
-                divergence_sum += dot(vel_diff, e_ij) * dW_ijV_j;
+                const Vec<R, D> Be = corrected ? b.correction[i] * e_ij : e_ij;
+                divergence_sum += dot(vel_diff, Be) * dW_ijV_j;
                 Vec<R, D> face_to_fluid_n = SGN(dot(e_ij, wb.normal[j])) * wb.normal[j];
                 R u_jump = dot(vel_diff, face_to_fluid_n);
                 p_dissipation += rs.DissipativePJump(u_jump) * dW_ijV_j * face_to_fluid_n;
@@ -418,6 +439,121 @@ static sphx_status summation(sphx_relation *inner, sphx_relation *const *contact
             }
         });
     }
+    return SPHX_OK;
+}
+
+// LinearCorrectionMatrix<Inner<WithUpdate>> + <Contact<>> (kernel_correction_ck.hpp:37-91):
+// inner interact (B = 0, B -= r_ij grad W_ij^T V_j), contact interact, then the update
+template <class R, int D>
+static sphx_status correction_matrix(sphx_relation *inner, sphx_relation *const *contact, int n_contact,
+                                     double alpha_d)
+{
+    Body<R, D> &b = as<R, D>(inner->source);
+    const R alpha = R(alpha_d);
+    NeighborKernel<R, D> nk(*b.kernel, b.h, b.h);
+    // neighbor_method.hpp:88-94 (nablaW_ij = dW(r) r / |r|), kernel_correction_ck.hpp:39-49
+    parallel_for(b.n, [&](size_t i) {
+        Mat<R, D> Bi = Mat<R, D>::Zero();
+        for (UInt n = inner->particle_offset[i]; n != inner->particle_offset[i + 1]; ++n)
+        {
+            UInt j = inner->neighbor_index[n];
+            Vec<R, D> r = b.pos[i] - b.pos[j];
+            Vec<R, D> nablaW_ijV_j = nk.dW(r) * normalized(r) * b.Vol[j];
+            Bi -= outer(r, nablaW_ijV_j);
+        }
+        b.correction[i] = Bi;
+    });
+    // kernel_correction_ck.hpp:82-91
+    for (int k = 0; k < n_contact; ++k)
+    {
+        sphx_relation *w = contact[k];
+        Body<R, D> &wb = as<R, D>(w->target);
+        NeighborKernel<R, D> wk(*b.kernel, b.h, wb.h);
+        parallel_for(b.n, [&](size_t i) {
+            for (UInt n = w->particle_offset[i]; n != w->particle_offset[i + 1]; ++n)
+            {
+                UInt j = w->neighbor_index[n];
+                Vec<R, D> r = b.pos[i] - wb.pos[j];
+                Vec<R, D> nablaW_ijV_j = wk.dW(r) * normalized(r) * wb.Vol[j];
+                b.correction[i] -= outer(r, nablaW_ijV_j);
+            }
+        });
+    }
+    // UpdateKernel::update, kernel_correction_ck.hpp:58-65
+    parallel_for(b.n, [&](size_t i) {
+        const Mat<R, D> &Bi = b.correction[i];
+        R determinant_i = determinant(Bi);
+        R det_sqr = std::max(alpha - determinant_i, R(0));
+        Mat<R, D> inv = inverseTikhonov(Bi, SqrtEps<R>());
+        R weight = determinant_i / (determinant_i + det_sqr);
+        b.correction[i] = weight * inv + (R(1.0) - weight) * Mat<R, D>::Identity();
+    });
+    return SPHX_OK;
+}
+
+// FreeSurfaceIndicationCK<Inner<WithUpdate>> + <Contact<>> (surface_indication_ck.hpp:11-178)
+template <class R, int D>
+static sphx_status surface_indication(sphx_relation *inner, sphx_relation *const *contact, int n_contact)
+{
+    Body<R, D> &b = as<R, D>(inner->source);
+    const R threshold = R(0.75) * R(D); // surface_indication_ck.hpp:19
+    const R smoothing_length = b.h;     // ReferenceSmoothingLength, :20
+    NeighborKernel<R, D> nk(*b.kernel, b.h, b.h);
+    auto first = [&](size_t i) { return inner->particle_offset[i]; };
+    auto last = [&](size_t i) { return inner->particle_offset[i + 1]; };
+    // inner InteractKernel::interact, :55-71, with isNearPreviousFreeSurface, :75-89
+    parallel_for(b.n, [&](size_t i) {
+        R pos_div = 0.0;
+        for (UInt n = first(i); n != last(i); ++n)
+        {
+            UInt j = inner->neighbor_index[n];
+            R r_ij = norm(b.pos[i] - b.pos[j]);
+            pos_div -= nk.dW(b.pos[i] - b.pos[j]) * b.Vol[j] * r_ij;
+        }
+        auto near_previous_surface = [&]() {
+            for (UInt n = first(i); n != last(i); ++n)
+                if (b.prev_indicator[inner->neighbor_index[n]] == 1)
+                    return true;
+            return false;
+        };
+        if (pos_div < threshold && b.prev_indicator[i] != 1 && !near_previous_surface())
+            pos_div = R(2.0) * threshold;
+        b.pos_div[i] = pos_div;
+    });
+    // contact InteractKernel::interact, :147-157
+    for (int k = 0; k < n_contact; ++k)
+    {
+        sphx_relation *w = contact[k];
+        Body<R, D> &wb = as<R, D>(w->target);
+        NeighborKernel<R, D> wk(*b.kernel, b.h, wb.h);
+        parallel_for(b.n, [&](size_t i) {
+            R pos_div = 0.0;
+            for (UInt n = w->particle_offset[i]; n != w->particle_offset[i + 1]; ++n)
+            {
+                UInt j = w->neighbor_index[n];
+                R r_ij = norm(b.pos[i] - wb.pos[j]);
+                pos_div -= wk.dW(b.pos[i] - wb.pos[j]) * wb.Vol[j] * r_ij;
+            }
+            b.pos_div[i] += pos_div;
+        });
+    }
+    // UpdateKernel::update, :100-109, with isVeryNearFreeSurface, :112-127
+    parallel_for(b.n, [&](size_t i) {
+        auto very_near_surface = [&]() {
+            for (UInt n = first(i); n != last(i); ++n)
+            {
+                UInt j = inner->neighbor_index[n];
+                R r_ij = norm(b.pos[i] - b.pos[j]);
+                if (b.pos_div[j] < threshold && r_ij < smoothing_length)
+                    return true;
+            }
+            return false;
+        };
+        b.indicator[i] = 1;
+        if (b.pos_div[i] > threshold && !very_near_surface())
+            b.indicator[i] = 0;
+        b.prev_indicator[i] = b.indicator[i];
+    });
     return SPHX_OK;
 }
 
@@ -485,6 +621,29 @@ static sphx_status rw_scalar(std::vector<R> &f, void *host, size_t n, bool write
         else
             h[i] = (double)f[i];
     }
+    return SPHX_OK;
+}
+template <class R, int D>
+static sphx_status rw_mat(std::vector<Mat<R, D>> &f, void *host, size_t n, bool write)
+{
+    double *h = (double *)host;
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b)
+            {
+                if (write)
+                    f[i].m[a][b] = R(h[(i * D + a) * D + b]);
+                else
+                    h[(i * D + a) * D + b] = (double)f[i].m[a][b];
+            }
+    return SPHX_OK;
+}
+static sphx_status rw_int(std::vector<int32_t> &f, void *host, size_t n, bool write)
+{
+    if (write)
+        std::memcpy(f.data(), host, n * sizeof(int32_t));
+    else
+        std::memcpy(host, f.data(), n * sizeof(int32_t));
     return SPHX_OK;
 }
 static sphx_status rw_uint(std::vector<UInt> &f, void *host, size_t n, bool write)
@@ -593,6 +752,10 @@ static sphx_status body_rw(sphx_body *b, int field, void *host, size_t n, bool w
         case SPHX_FIELD_SORTED_ID: return rw_uint(B.sorted_id, host, n, write);
         case SPHX_FIELD_SORT_SEQUENCE: return rw_uint(B.sequence, host, n, write);
         case SPHX_FIELD_INTERPOLATED: return rw_scalar(B.interp, host, n, write);
+        case SPHX_FIELD_CORRECTION_MATRIX: return rw_mat(B.correction, host, n, write);
+        case SPHX_FIELD_INDICATOR: return rw_int(B.indicator, host, n, write);
+        case SPHX_FIELD_PREVIOUS_INDICATOR: return rw_int(B.prev_indicator, host, n, write);
+        case SPHX_FIELD_POSITION_DIVERGENCE: return rw_scalar(B.pos_div, host, n, write);
         default: return fail(SPHX_E_ARG, "unknown field");
         }
     });
@@ -769,6 +932,8 @@ sphx_status sphx_particle_sort(sphx_body *body, sphx_cell_linked_list *cll)
         // force_prior_ck.cpp:13, density_regularization.hpp:24)
         gather(B.pos), gather(B.Vol), gather(B.original_id), gather(B.vel), gather(B.mass);
         gather(B.force_prior), gather(B.C), gather(B.dC), gather(B.Vol_ref);
+        // evolving once the free-surface indication exists (surface_indication_ck.hpp:43)
+        gather(B.prev_indicator);
         for (UInt i = 0; i < B.n; ++i)
             B.sorted_id[B.original_id[i]] = i;
         return SPHX_OK;
@@ -843,24 +1008,63 @@ sphx_status sphx_compression_summation(sphx_relation *inner, sphx_relation *cons
         return summation<R, D>(inner, contact, n);
     });
 }
-sphx_status sphx_acoustic_step_1st_half(sphx_relation *inner, sphx_relation *const *walls, int n,
-                                        int riemann, double dt)
+static sphx_status first_half(sphx_relation *inner, sphx_relation *const *walls, int n, int riemann, double dt,
+                              bool corrected)
 {
     if (!relations_ok(inner, walls, n))
         return fail(SPHX_E_ARG, "acoustic step needs updated relations of one source");
     return with_body(inner->source, [&](auto &B) {
         BODY_TYPES(B);
-        return acoustic_1st<R, D>(inner, walls, n, riemann, dt);
+        return acoustic_1st<R, D>(inner, walls, n, riemann, dt, corrected);
     });
+}
+static sphx_status second_half(sphx_relation *inner, sphx_relation *const *walls, int n, int riemann, double dt,
+                               bool corrected)
+{
+    if (!relations_ok(inner, walls, n))
+        return fail(SPHX_E_ARG, "acoustic step needs updated relations of one source");
+    return with_body(inner->source, [&](auto &B) {
+        BODY_TYPES(B);
+        return acoustic_2nd<R, D>(inner, walls, n, riemann, dt, corrected);
+    });
+}
+sphx_status sphx_acoustic_step_1st_half(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                        int riemann, double dt)
+{
+    return first_half(inner, walls, n, riemann, dt, false);
 }
 sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *const *walls, int n,
                                         int riemann, double dt)
 {
-    if (!relations_ok(inner, walls, n))
-        return fail(SPHX_E_ARG, "acoustic step needs updated relations of one source");
+    return second_half(inner, walls, n, riemann, dt, false);
+}
+sphx_status sphx_acoustic_step_1st_half_corrected(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                                  int riemann, double dt)
+{
+    return first_half(inner, walls, n, riemann, dt, true);
+}
+sphx_status sphx_acoustic_step_2nd_half_corrected(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                                  int riemann, double dt)
+{
+    return second_half(inner, walls, n, riemann, dt, true);
+}
+sphx_status sphx_linear_correction_matrix(sphx_relation *inner, sphx_relation *const *contact, int n,
+                                          double alpha)
+{
+    if (!relations_ok(inner, contact, n))
+        return fail(SPHX_E_ARG, "correction matrix needs updated relations of one source");
     return with_body(inner->source, [&](auto &B) {
         BODY_TYPES(B);
-        return acoustic_2nd<R, D>(inner, walls, n, riemann, dt);
+        return correction_matrix<R, D>(inner, contact, n, alpha);
+    });
+}
+sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *const *contact, int n)
+{
+    if (!relations_ok(inner, contact, n))
+        return fail(SPHX_E_ARG, "surface indication needs updated relations of one source");
+    return with_body(inner->source, [&](auto &B) {
+        BODY_TYPES(B);
+        return surface_indication<R, D>(inner, contact, n);
     });
 }
 
@@ -979,6 +1183,12 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     double adv_dt = 0;
     if ((s = sphx_advection_time_step(fluid, cfg->u_ref, cfg->advection_cfl, &adv_dt)))
         return s;
+    // dambreak.cpp:192-193
+    if (cfg->surface_indication && (s = sphx_free_surface_indication(inner, walls, n_wall)))
+        return s;
+    const bool corrected = cfg->correction == SPHX_CORRECTION_LINEAR;
+    if (corrected && (s = sphx_linear_correction_matrix(inner, walls, n_wall, cfg->correction_alpha)))
+        return s;
     double relaxation_time = 0.0;
     uint32_t count = 0;
     while (relaxation_time < adv_dt)
@@ -986,9 +1196,9 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
         double dt = 0;
         if ((s = sphx_acoustic_time_step(fluid, cfg->acoustic_cfl, &dt)))
             return s;
-        if ((s = sphx_acoustic_step_1st_half(inner, walls, n_wall, cfg->riemann, dt)))
+        if ((s = first_half(inner, walls, n_wall, cfg->riemann, dt, corrected)))
             return s;
-        if ((s = sphx_acoustic_step_2nd_half(inner, walls, n_wall, cfg->riemann, dt)))
+        if ((s = second_half(inner, walls, n_wall, cfg->riemann, dt, corrected)))
             return s;
         relaxation_time += dt;
         *time += dt;

@@ -155,12 +155,163 @@ inline Vec<R, D> normalized(const Vec<R, D> &a)
     R z = squaredNorm(a);
     return z > R(0) ? a / R(std::sqrt(z)) : a;
 }
+//---------------------------------------------------------------------------
+// small fixed matrices (Eigen Matd stand-in), row-major m[row][col]
+//---------------------------------------------------------------------------
+template <class R, int D>
+struct Mat
+{
+    R m[D][D];
+    static Mat Zero()
+    {
+        Mat r;
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b)
+                r.m[a][b] = R(0);
+        return r;
+    }
+    static Mat Identity()
+    {
+        Mat r = Zero();
+        for (int a = 0; a < D; ++a)
+            r.m[a][a] = R(1);
+        return r;
+    }
+    Mat transpose() const
+    {
+        Mat r;
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b)
+                r.m[a][b] = m[b][a];
+        return r;
+    }
+};
+template <class R, int D>
+inline Mat<R, D> operator+(const Mat<R, D> &x, const Mat<R, D> &y)
+{
+    Mat<R, D> r;
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+            r.m[a][b] = x.m[a][b] + y.m[a][b];
+    return r;
+}
+template <class R, int D>
+inline Mat<R, D> operator*(const Mat<R, D> &x, R s)
+{
+    Mat<R, D> r;
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+            r.m[a][b] = x.m[a][b] * s;
+    return r;
+}
+template <class R, int D>
+inline Mat<R, D> operator*(R s, const Mat<R, D> &x)
+{
+    Mat<R, D> r;
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+            r.m[a][b] = s * x.m[a][b];
+    return r;
+}
+// matrix product; the inner sum runs in index order
+template <class R, int D>
+inline Mat<R, D> operator*(const Mat<R, D> &x, const Mat<R, D> &y)
+{
+    Mat<R, D> r;
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+        {
+            R s = x.m[a][0] * y.m[0][b];
+            for (int k = 1; k < D; ++k)
+                s += x.m[a][k] * y.m[k][b];
+            r.m[a][b] = s;
+        }
+    return r;
+}
+template <class R, int D>
+inline Vec<R, D> operator*(const Mat<R, D> &x, const Vec<R, D> &v)
+{
+    Vec<R, D> r;
+    for (int a = 0; a < D; ++a)
+    {
+        R s = x.m[a][0] * v[0];
+        for (int k = 1; k < D; ++k)
+            s += x.m[a][k] * v[k];
+        r[a] = s;
+    }
+    return r;
+}
+template <class R, int D>
+inline Mat<R, D> &operator-=(Mat<R, D> &x, const Mat<R, D> &y)
+{
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+            x.m[a][b] -= y.m[a][b];
+    return x;
+}
+// outer product u v^T
+template <class R, int D>
+inline Mat<R, D> outer(const Vec<R, D> &u, const Vec<R, D> &v)
+{
+    Mat<R, D> r;
+    for (int a = 0; a < D; ++a)
+        for (int b = 0; b < D; ++b)
+            r.m[a][b] = u[a] * v[b];
+    return r;
+}
+// determinant by cofactor expansion along the first column
+template <class R, int D>
+inline R determinant(const Mat<R, D> &x)
+{
+    if (D == 2)
+        return x.m[0][0] * x.m[1][1] - x.m[1][0] * x.m[0][1];
+    auto minor = [&](int a, int b, int c) {
+        return x.m[a][0] * (x.m[b][1] * x.m[c][2] - x.m[b][2] * x.m[c][1]);
+    };
+    return minor(0, 1, 2) - minor(1, 0, 2) + minor(2, 0, 1);
+}
+// inverse as adjugate / determinant (the determinant from the first-column cofactors)
+template <class R, int D>
+inline Mat<R, D> inverse(const Mat<R, D> &x)
+{
+    Mat<R, D> r;
+    if (D == 2)
+    {
+        const R inv_det = R(1) / (x.m[0][0] * x.m[1][1] - x.m[1][0] * x.m[0][1]);
+        r.m[0][0] = x.m[1][1] * inv_det;
+        r.m[1][0] = -x.m[1][0] * inv_det;
+        r.m[0][1] = -x.m[0][1] * inv_det;
+        r.m[1][1] = x.m[0][0] * inv_det;
+        return r;
+    }
+    // cofactor of entry (i, j) with cyclic indices
+    auto cof = [&](int i, int j) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        return x.m[i1][j1] * x.m[i2][j2] - x.m[i1][j2] * x.m[i2][j1];
+    };
+    const R det = cof(0, 0) * x.m[0][0] + cof(1, 0) * x.m[1][0] + cof(2, 0) * x.m[2][0];
+    const R inv_det = R(1) / det;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            r.m[a][b] = cof(b, a) * inv_det;
+    return r;
+}
+// Tikhonov-regularised inverse (B^T B + eps I)^-1 B^T, common/vector_functions.h:198-203
+template <class R, int D>
+inline Mat<R, D> inverseTikhonov(const Mat<R, D> &x, R eps)
+{
+    const Mat<R, D> xt = x.transpose();
+    return inverse(xt * x + eps * Mat<R, D>::Identity()) * xt;
+}
+
 // src/shared/common/scalar_functions.h:128-131
 template <class R>
 inline R SGN(R x) { return (x < 0) ? R(-1) : ((x > 0) ? R(1) : R(0)); }
 
 template <class R>
 constexpr R TinyReal() { return R(2.71051e-20); } // base_data_type.h:207
+template <class R>
+constexpr R SqrtEps() { return R(1.0e-8); } // base_data_type.h:206
 
 //---------------------------------------------------------------------------
 // Wendland C2 kernel and its 24-entry tabulation

@@ -13,6 +13,7 @@ FP64, MIXED, FP32 = 0, 1, 2
 BODY_FLUID, BODY_SOLID, BODY_OBSERVER = 0, 1, 2
 RIEMANN_ACOUSTIC, RIEMANN_NONE, RIEMANN_DISSIPATIVE = 0, 1, 2
 FLOW_INTERNAL, FLOW_FREE_SURFACE = 0, 1
+CORRECTION_NONE, CORRECTION_LINEAR = 0, 1
 
 FIELDS = {
     "Position": (0, "vec"), "VolumetricMeasure": (1, "real"), "Mass": (2, "real"),
@@ -23,6 +24,8 @@ FIELDS = {
     "NormalDirection": (13, "vec"), "WallVelocity": (14, "vec"),
     "WallAcceleration": (15, "vec"), "OriginalID": (16, "uint"), "SortedID": (17, "uint"),
     "SortSequence": (18, "uint"), "Interpolated": (19, "real"),
+    "LinearCorrectionMatrix": (20, "mat"), "Indicator": (21, "int"),
+    "PreviousSurfaceIndicator": (22, "int"), "PositionDivergence": (23, "real"),
 }
 
 # every symbol include/sphx.h declares
@@ -39,7 +42,9 @@ SYMBOLS = [
     "sphx_acoustic_step_1st_half", "sphx_acoustic_step_2nd_half", "sphx_acoustic_time_step",
     "sphx_advection_time_step", "sphx_total_mechanical_energy", "sphx_upper_front_in_axis",
     "sphx_interpolate_contact", "sphx_advection_step", "sphx_profile_enable",
-    "sphx_profile_read",
+    "sphx_profile_read", "sphx_linear_correction_matrix",
+    "sphx_acoustic_step_1st_half_corrected", "sphx_acoustic_step_2nd_half_corrected",
+    "sphx_free_surface_indication",
 ]
 
 
@@ -52,7 +57,8 @@ class BodyDesc(C.Structure):
 class AdvectionConfig(C.Structure):
     _fields_ = [("u_ref", C.c_double), ("advection_cfl", C.c_double),
                 ("acoustic_cfl", C.c_double), ("riemann", C.c_int), ("flow_type", C.c_int),
-                ("sort_now", C.c_int)]
+                ("sort_now", C.c_int), ("correction", C.c_int), ("correction_alpha", C.c_double),
+                ("surface_indication", C.c_int)]
 
 
 class SphxError(RuntimeError):
@@ -110,6 +116,10 @@ class SphxLibrary:
                                               _dp, _up]),
             "sphx_profile_enable": (C.c_int, [_vp, C.c_int]),
             "sphx_profile_read": (C.c_int, [_vp, C.c_char_p, _dp, C.POINTER(C.c_uint64)]),
+            "sphx_linear_correction_matrix": (C.c_int, [_vp, _pp, C.c_int, C.c_double]),
+            "sphx_acoustic_step_1st_half_corrected": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
+            "sphx_acoustic_step_2nd_half_corrected": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
+            "sphx_free_surface_indication": (C.c_int, [_vp, _pp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

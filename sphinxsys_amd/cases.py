@@ -94,8 +94,10 @@ class DamBreak:
     """Bodies, relations and CK dynamics of one dam-break scene on one library."""
 
     def __init__(self, lib, geom, precision=capi.FP64, riemann=capi.RIEMANN_ACOUSTIC, device=0,
-                 ctx=None):
+                 ctx=None, correction=capi.CORRECTION_NONE, correction_alpha=0.5,
+                 surface_indication=False):
         self.lib, self.geom, self.precision = lib, geom, precision
+        self.correction, self.surface_indication = correction, surface_indication
         d = geom.dim
         self.own_ctx = ctx is None
         self.ctx = ctx if ctx is not None else ck.Context(lib, device)
@@ -130,15 +132,22 @@ class DamBreak:
         self.regularization = ck.DensityRegularization(self.fluid, capi.FLOW_FREE_SURFACE)
         self.setup = ck.AdvectionStepSetup(self.fluid)
         self.position = ck.UpdateParticlePosition(self.fluid)
-        self.first_half = ck.AcousticStep1stHalf(self.inner, self.contact, riemann=riemann)
-        self.second_half = ck.AcousticStep2ndHalf(self.inner, self.contact, riemann=riemann)
+        self.first_half = ck.AcousticStep1stHalf(self.inner, self.contact, riemann=riemann,
+                                                 correction=correction)
+        self.second_half = ck.AcousticStep2ndHalf(self.inner, self.contact, riemann=riemann,
+                                                  correction=correction)
+        self.correction_matrix = ck.LinearCorrectionMatrixComplex(self.inner, self.contact,
+                                                                  alpha=correction_alpha)
+        self.boundary_indicator = ck.FreeSurfaceIndicationComplexSpatialTemporalCK(self.inner, self.contact)
         self.acoustic_dt = ck.AcousticTimeStepCK(self.fluid)
         self.advection_dt = ck.AdvectionTimeStepCK(self.fluid, geom.u_ref)
         self.energy = ck.TotalMechanicalEnergyCK(self.fluid, geom.gravity)
         self.probe = ck.InterpolationContact(self.observer_contact, "Pressure")
         self.front = ck.UpperFrontInAxisDirectionCK(self.fluid, 0)
         self.advection = ck.AdvectionStep(self.inner, [self.contact], self.fluid_cll, geom.u_ref,
-                                          riemann=riemann)
+                                          riemann=riemann, correction=correction,
+                                          correction_alpha=correction_alpha,
+                                          surface_indication=surface_indication)
         self.time = 0.0
         self.iterations = 0
         self.acoustic_steps = 0
@@ -171,6 +180,10 @@ class DamBreak:
         self.regularization.exec()
         self.setup.exec()
         adv_dt = self.advection_dt.exec()
+        if self.surface_indication:
+            self.boundary_indicator.exec()
+        if self.correction == capi.CORRECTION_LINEAR:
+            self.correction_matrix.exec()
         relaxation, n = 0.0, 0
         while relaxation < adv_dt:
             dt = self.acoustic_dt.exec()

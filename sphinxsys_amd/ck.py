@@ -12,6 +12,8 @@ Class names, constructor arguments and exec() semantics follow the reference:
                         fluid_dynamics/fluid_time_step_ck.h:42-184
   TotalMechanicalEnergyCK, UpperFrontInAxisDirectionCK  general_dynamics/general_reduce_ck.h:69-222
   Interpolation<Contact<Real>>                  general_dynamics/interpolation_dynamics.hpp:46-60
+  LinearCorrectionMatrix                        general_dynamics/kernel_correction_ck.h:40-120
+  FreeSurfaceIndicationCK                       general_dynamics/surface_indication/surface_indication_ck.h
 Every object takes the SphxLibrary it runs on; the product binding is
 `sphinxsys_amd.load()` (HIP).  Errors raise SphxError with the library's message.
 """
@@ -70,6 +72,10 @@ class Body:
         fid, kind = capi.FIELDS[name]
         if kind == "uint":
             a = np.ascontiguousarray(values, dtype=np.uint32).reshape(self.n)
+        elif kind == "int":
+            a = np.ascontiguousarray(values, dtype=np.int32).reshape(self.n)
+        elif kind == "mat":
+            a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n, self.dim, self.dim)
         elif kind == "vec":
             a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n, self.dim)
         else:
@@ -80,6 +86,10 @@ class Body:
         fid, kind = capi.FIELDS[name]
         if kind == "uint":
             a = np.empty(self.n, dtype=np.uint32)
+        elif kind == "int":
+            a = np.empty(self.n, dtype=np.int32)
+        elif kind == "mat":
+            a = np.empty((self.n, self.dim, self.dim), dtype=np.float64)
         elif kind == "vec":
             a = np.empty((self.n, self.dim), dtype=np.float64)
         else:
@@ -238,21 +248,22 @@ class CompressionSummation(_FluidInteraction):
 
 
 class AcousticStep1stHalf(_FluidInteraction):
-    """AcousticStep1stHalf<Inner<OneLevel, R, NoKernelCorrectionCK>, Contact<Wall, R, ...>>."""
+    """AcousticStep1stHalf<Inner<OneLevel, R, K>, Contact<Wall, R, K>> with the kernel
+    correction K = NoKernelCorrectionCK (correction=CORRECTION_NONE) or LinearCorrectionCK."""
 
-    def __init__(self, inner, *walls, riemann=capi.RIEMANN_ACOUSTIC):
+    _entry = ("sphx_acoustic_step_1st_half", "sphx_acoustic_step_1st_half_corrected")
+
+    def __init__(self, inner, *walls, riemann=capi.RIEMANN_ACOUSTIC, correction=capi.CORRECTION_NONE):
         super().__init__(inner, *walls)
-        self.riemann = riemann
+        self.riemann, self.correction = riemann, correction
 
     def exec(self, dt=0.0):
-        self.lib.call("sphx_acoustic_step_1st_half", self.inner.h, self._ptrs, len(self.contacts),
-                      self.riemann, float(dt))
+        self.lib.call(self._entry[1 if self.correction == capi.CORRECTION_LINEAR else 0], self.inner.h,
+                      self._ptrs, len(self.contacts), self.riemann, float(dt))
 
 
 class AcousticStep2ndHalf(AcousticStep1stHalf):
-    def exec(self, dt=0.0):
-        self.lib.call("sphx_acoustic_step_2nd_half", self.inner.h, self._ptrs, len(self.contacts),
-                      self.riemann, float(dt))
+    _entry = ("sphx_acoustic_step_2nd_half", "sphx_acoustic_step_2nd_half_corrected")
 
 
 def AcousticStep1stHalfWithWallRiemannCK(inner, *walls):
@@ -261,6 +272,37 @@ def AcousticStep1stHalfWithWallRiemannCK(inner, *walls):
 
 def AcousticStep2ndHalfWithWallRiemannCK(inner, *walls):
     return AcousticStep2ndHalf(inner, *walls, riemann=capi.RIEMANN_ACOUSTIC)
+
+
+def AcousticStep1stHalfWithWallRiemannCorrectionCK(inner, *walls):
+    return AcousticStep1stHalf(inner, *walls, riemann=capi.RIEMANN_ACOUSTIC,
+                               correction=capi.CORRECTION_LINEAR)
+
+
+def AcousticStep2ndHalfWithWallRiemannCorrectionCK(inner, *walls):
+    return AcousticStep2ndHalf(inner, *walls, riemann=capi.RIEMANN_ACOUSTIC,
+                               correction=capi.CORRECTION_LINEAR)
+
+
+class LinearCorrectionMatrixComplex(_FluidInteraction):
+    """LinearCorrectionMatrix<Inner<WithUpdate>, Contact<>> built with
+    DynamicsArgs(inner, alpha) and the contact relations."""
+
+    def __init__(self, inner, *contacts, alpha=0.0):
+        super().__init__(inner, *contacts)
+        self.alpha = alpha
+
+    def exec(self, dt=0.0):
+        self.lib.call("sphx_linear_correction_matrix", self.inner.h, self._ptrs, len(self.contacts),
+                      float(self.alpha))
+
+
+class FreeSurfaceIndicationComplexSpatialTemporalCK(_FluidInteraction):
+    """FreeSurfaceIndicationCK<Inner<WithUpdate>, Contact<>>: "Indicator",
+    "PositionDivergence" and "PreviousSurfaceIndicator" of the inner body."""
+
+    def exec(self, dt=0.0):
+        self.lib.call("sphx_free_surface_indication", self.inner.h, self._ptrs, len(self.contacts))
 
 
 class AcousticTimeStepCK:
@@ -322,9 +364,11 @@ class AdvectionStep:
     """The dual-time advection step of the CK dam-break loop behind one call."""
 
     def __init__(self, inner, walls, cll, u_ref, riemann=capi.RIEMANN_ACOUSTIC,
-                 flow=capi.FLOW_FREE_SURFACE, advection_cfl=0.25, acoustic_cfl=0.6):
+                 flow=capi.FLOW_FREE_SURFACE, advection_cfl=0.25, acoustic_cfl=0.6,
+                 correction=capi.CORRECTION_NONE, correction_alpha=0.5, surface_indication=False):
         self.inner, self.walls, self.cll, self.lib = inner, walls, cll, inner.lib
-        self.cfg = capi.AdvectionConfig(u_ref, advection_cfl, acoustic_cfl, riemann, flow, 0)
+        self.cfg = capi.AdvectionConfig(u_ref, advection_cfl, acoustic_cfl, riemann, flow, 0,
+                                        correction, correction_alpha, 1 if surface_indication else 0)
         self._ptrs, self._keep = capi.SphxLibrary.ptr_array([w.h for w in walls])
 
     def exec(self, physical_time, sort_now=False):

@@ -84,6 +84,13 @@ struct FluidArgs
     const KernelPoly<T> *poly;
     int n_wall;
     WallArgs<T> wall[kMaxWalls];
+    // LinearCorrectionMatrix components (a, b) at corr[a * D + b]; grad_sum =
+    // sum_j B_j e_ij dW_ij V_j of the inner neighbours
+    T *corr[9];
+    T *grad_sum[3];
+    // free-surface indication state
+    int32_t *indicator, *prev_indicator;
+    T *pos_div;
 };
 
 // position/volume record of particle j
@@ -200,11 +207,45 @@ __global__ void __launch_bounds__(kThreads) k_init_2nd(FluidArgs<typename Prec<P
         a.dpos[2][i] += (double)(v.z * dt * T(0.5));
 }
 
+// B v for the correction matrix of particle i (components corr[a * D + b])
+template <int D, class T>
+struct Mat3
+{
+    T m[D][D];
+    __device__ __forceinline__ void load(T *const *c, uint32_t i)
+    {
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int q = 0; q < D; ++q)
+                m[r][q] = c[r * D + q][i];
+    }
+    __device__ __forceinline__ void apply(T x, T y, T z, T &ox, T &oy, T &oz) const
+    {
+        if (D == 3)
+        {
+            ox = m[0][0] * x + m[0][1] * y + m[0][2] * z;
+            oy = m[1][0] * x + m[1][1] * y + m[1][2] * z;
+            oz = m[2][0] * x + m[2][1] * y + m[2][2] * z;
+        }
+        else
+        {
+            ox = m[0][0] * x + m[0][1] * y;
+            oy = m[1][0] * x + m[1][1] * y;
+            oz = T(0);
+        }
+    }
+};
+
 //---------------------------------------------------------------------------
 // pass 1: inner + wall interact of AcousticStep1stHalf, then its update
 // (v += (F_prior + F) / m dt); optionally the 2nd half's initialize.
+// CORR (LinearCorrectionCK): the inner pair term 2 AverageP(B_j p_i, B_i p_j) e_ij
+// = k (p_i B_j + p_j B_i) e_ij is summed as p_i grad_sum_i + B_i sum_j p_j dW V_j e_ij
+// (grad_sum_i = sum_j B_j e_ij dW V_j is fixed while positions and volumes are),
+// the wall term as B_i sum_w (p_i + p_w) dW V_w e.
 //---------------------------------------------------------------------------
-template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND>
+template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
 __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
 {
     using T = typename Prec<P>::T;
@@ -245,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
         const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
         const T ir = inv_sqrt(r2);
         const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
-        const T s = (p_i + p_j) * dWV * ir;
+        const T s = (CORR ? p_j : p_i + p_j) * dWV * ir;
         fx += s * dx;
         fy += s * dy;
         if (D == 3)
@@ -261,6 +302,17 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     const T k_avg = T(2) * a.inv_rho0c0_sum * a.rho0c0; // 2 * AverageP / (p_i + p_j)
     const T vol_i = pi.vol;
     T Fx = a.force[0][i], Fy = a.force[1][i], Fz = D == 3 ? a.force[2][i] : T(0);
+    Mat3<D, T> Bi;
+    if (CORR)
+    {
+        Bi.load(a.corr, i);
+        T bx, by, bz;
+        Bi.apply(fx, fy, fz, bx, by, bz);
+        fx = p_i * a.grad_sum[0][i] + bx;
+        fy = p_i * a.grad_sum[1][i] + by;
+        if (D == 3)
+            fz = p_i * a.grad_sum[2][i] + bz;
+    }
     Fx += -k_avg * fx * vol_i;
     Fy += -k_avg * fy * vol_i;
     if (D == 3)
@@ -301,6 +353,12 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
                     wz += s * ez;
                 if (RIEMANN)
                     wcd += (p_i - p_w) * dWV;
+            }
+            if (CORR)
+            {
+                T bx, by, bz;
+                Bi.apply(wx, wy, wz, bx, by, bz);
+                wx = bx, wy = by, wz = bz;
             }
             Fx += -wx * vol_i;
             Fy += -wy * vol_i;
@@ -346,7 +404,8 @@ __device__ __forceinline__ T p_jump_scaled(const FluidArgs<T> &a, T u)
 // (C += dt/2 dC, rho = C rho0) and, fused, the per-particle signal speed of
 // AcousticTimeStepCK for the next sub-step, max-reduced into the max lanes at max_slot.
 //---------------------------------------------------------------------------
-template <int D, int P, bool RIEMANN, bool FUSE_CFL>
+// CORR (LinearCorrectionCK): the divergence uses B_i e_ij in place of e_ij
+template <int D, int P, bool RIEMANN, bool FUSE_CFL, bool CORR>
 __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d,
                                                     unsigned long long *max_slot)
 {
@@ -383,6 +442,9 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         const T x_per_r = a.poly->x_per_r;
         const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
         const Rec4<T> vi = a.vel[i];
+        Mat3<D, T> Bi;
+        if (CORR)
+            Bi.load(a.corr, i);
         // divergence: 2 (v_i - v_ave).e dW V = (v_i - v_j).e dW V for one fluid
         T div = 0, px = 0, py = 0, pz = 0;
         auto pair = [&](const PV<T, D> &pj, const Rec4<T> &vj) {
@@ -394,7 +456,17 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
             T u = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey;
             if (D == 3)
                 u += (vi.z - vj.z) * ez;
-            div += u * dWV;
+            if (CORR)
+            {
+                T bx, by, bz;
+                Bi.apply(ex, ey, ez, bx, by, bz);
+                T uc = (vi.x - vj.x) * bx + (vi.y - vj.y) * by;
+                if (D == 3)
+                    uc += (vi.z - vj.z) * bz;
+                div += uc * dWV;
+            }
+            else
+                div += u * dWV;
             if (RIEMANN)
             {
                 const T s = p_jump_scaled(a, u) * dWV;
@@ -431,9 +503,12 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
                 const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
                 const T vdx = T(2) * (vi.x - w.vel[0][j]), vdy = T(2) * (vi.y - w.vel[1][j]);
                 const T vdz = D == 3 ? T(2) * (vi.z - w.vel[2][j]) : T(0);
-                T ve = vdx * ex + vdy * ey;
+                T bex = ex, bey = ey, bez = ez;
+                if (CORR)
+                    Bi.apply(ex, ey, ez, bex, bey, bez);
+                T ve = vdx * bex + vdy * bey;
                 if (D == 3)
-                    ve += vdz * ez;
+                    ve += vdz * bez;
                 wdiv += ve * dWV;
                 if (RIEMANN)
                 {
@@ -567,6 +642,256 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
         }
     }
     a.S[i] = s;
+}
+
+//---------------------------------------------------------------------------
+// LinearCorrectionMatrix<Inner<WithUpdate>> + <Contact<>> (kernel_correction_ck.hpp:37-91).
+// B_raw = -sum_j r_ij (x) grad W_ij V_j = -sum_j (dW_ij V_j / |r_ij|) r_ij r_ij^T is
+// symmetric, so six (three in 2D) sums are kept; the update (det, Tikhonov
+// inverse, blend with I) runs in double per particle in both precisions.
+//---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ double det_d(const double (&m)[3][3])
+{
+    if (D == 2)
+        return m[0][0] * m[1][1] - m[1][0] * m[0][1];
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[1][0] * (m[0][1] * m[2][2] - m[0][2] * m[2][1]) +
+           m[2][0] * (m[0][1] * m[1][2] - m[0][2] * m[1][1]);
+}
+// adjugate / determinant, the determinant from the first-column cofactors
+template <int D>
+__device__ __forceinline__ void inverse_d(const double (&x)[3][3], double (&r)[3][3])
+{
+    if (D == 2)
+    {
+        const double inv_det = 1.0 / (x[0][0] * x[1][1] - x[1][0] * x[0][1]);
+        r[0][0] = x[1][1] * inv_det;
+        r[1][0] = -x[1][0] * inv_det;
+        r[0][1] = -x[0][1] * inv_det;
+        r[1][1] = x[0][0] * inv_det;
+        return;
+    }
+    double cof[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            cof[i][j] = x[i1][j1] * x[i2][j2] - x[i1][j2] * x[i2][j1];
+        }
+    const double det = cof[0][0] * x[0][0] + cof[1][0] * x[1][0] + cof[2][0] * x[2][0];
+    const double inv_det = 1.0 / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r[i][j] = cof[j][i] * inv_det;
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_correction(FluidArgs<typename Prec<P>::T> a, double alpha)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    extern __shared__ __align__(32) unsigned char staged_lds[];
+    stage_poly(poly_in, a.poly, true);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, true);
+    const uint32_t blk = xcd_block_index();
+    const uint32_t i = blk * kThreads + threadIdx.x;
+    Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
+    const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
+    const bool staged = n_staged != kStageOverflow;
+    if (staged)
+    {
+        const uint32_t *keys = a.stage_keys + (size_t)blk * a.stage_cap;
+        for (uint32_t r = threadIdx.x; r < n_staged; r += kThreads)
+            s_pv[r] = a.posvol[keys[r]];
+    }
+    __syncthreads();
+    if (i >= a.n)
+        return;
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    // b_ab = sum dW V / |r| r_a r_b (B_raw = -b)
+    T bxx = 0, bxy = 0, bxz = 0, byy = 0, byz = 0, bzz = 0;
+    auto acc = [&](const PV<T, D> &pj, const PolyLDS<T> &poly, T x_per_r) {
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T ir = inv_sqrt(r2);
+        const T s = eval_poly(poly, r2 * ir, x_per_r) * pj.vol * ir;
+        const T sx = s * dx, sy = s * dy;
+        bxx += sx * dx;
+        bxy += sx * dy;
+        byy += sy * dy;
+        if (D == 3)
+        {
+            const T sz = s * dz;
+            bxz += sx * dz;
+            byz += sy * dz;
+            bzz += sz * dz;
+        }
+    };
+    const T x_per_r = a.poly->x_per_r;
+    const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+    if (staged)
+        for_each_neighbor(a.local_slice, nb, nc, [&](uint32_t r) { acc(load_pv<T, D>(s_pv, r), poly_in, x_per_r); });
+    else
+        for_each_neighbor(a.slice, nb, nc, [&](uint32_t j) { acc(load_pv<T, D>(a.posvol, j), poly_in, x_per_r); });
+    for (int k = 0; k < a.n_wall; ++k)
+    {
+        const WallArgs<T> &w = a.wall[k];
+        const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+        for (uint32_t k2 = 0; k2 < mc; ++k2)
+            acc(load_pv<T, D>(w.posvol, w.slice[mb + (k2 << 6)]), poly_w[k], w.poly->x_per_r);
+    }
+    // UpdateKernel::update (kernel_correction_ck.hpp:58-65), in double
+    double B[3][3] = {{-(double)bxx, -(double)bxy, -(double)bxz},
+                      {-(double)bxy, -(double)byy, -(double)byz},
+                      {-(double)bxz, -(double)byz, -(double)bzz}};
+    const double det = det_d<D>(B);
+    const double det_sqr = fmax(alpha - det, 0.0);
+    // inverseTikhonov: (B^T B + 1e-8 I)^-1 B^T (common/vector_functions.h:198-203)
+    double A[3][3], Ai[3][3];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+        {
+            double t = B[0][r] * B[0][q];
+#pragma unroll
+            for (int k = 1; k < D; ++k)
+                t += B[k][r] * B[k][q];
+            A[r][q] = t + (r == q ? 1.0e-8 : 0.0);
+        }
+    inverse_d<D>(A, Ai);
+    const double weight = det / (det + det_sqr);
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int q = 0; q < D; ++q)
+        {
+            double t = Ai[r][0] * B[q][0];
+#pragma unroll
+            for (int k = 1; k < D; ++k)
+                t += Ai[r][k] * B[q][k];
+            a.corr[r * D + q][i] = T(weight * t + (1.0 - weight) * (r == q ? 1.0 : 0.0));
+        }
+}
+
+// grad_sum_i = sum_j B_j e_ij dW_ij V_j over the inner neighbours (the part of
+// the corrected pressure force that does not change within an advection step)
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_correction_grad(FluidArgs<typename Prec<P>::T> a)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    stage_poly(poly_in, a.poly, true);
+    __syncthreads();
+    const uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    const T x_per_r = a.poly->x_per_r;
+    T gx = 0, gy = 0, gz = 0;
+    for_each_neighbor(a.slice, a.chunk[i >> 6] + (i & 63u), a.count[i], [&](uint32_t j) {
+        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T ir = inv_sqrt(r2);
+        const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
+        Mat3<D, T> Bj;
+        Bj.load(a.corr, j);
+        T bx, by, bz;
+        Bj.apply(dx * ir, dy * ir, D == 3 ? dz * ir : T(0), bx, by, bz);
+        gx += bx * dWV;
+        gy += by * dWV;
+        if (D == 3)
+            gz += bz * dWV;
+    });
+    a.grad_sum[0][i] = gx;
+    a.grad_sum[1][i] = gy;
+    if (D == 3)
+        a.grad_sum[2][i] = gz;
+}
+
+//---------------------------------------------------------------------------
+// FreeSurfaceIndicationCK<Inner<WithUpdate>> + <Contact<>>
+// (surface_indication/surface_indication_ck.hpp:55-157): position divergence
+// -sum dW V r, held at twice the threshold for a particle that neither was nor
+// neighbours the previous free surface, then the contact part ...
+//---------------------------------------------------------------------------
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_surface_div(FluidArgs<typename Prec<P>::T> a)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    stage_poly(poly_in, a.poly, true);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, true);
+    __syncthreads();
+    const uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T threshold = T(0.75) * T(D);
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    auto term = [&](const PV<T, D> &pj, const PolyLDS<T> &poly, T x_per_r) {
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r = sqrt(D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy);
+        return eval_poly(poly, r, x_per_r) * pj.vol * r;
+    };
+    const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+    T pos_div = 0;
+    for_each_neighbor(a.slice, nb, nc,
+                      [&](uint32_t j) { pos_div -= term(load_pv<T, D>(a.posvol, j), poly_in, a.poly->x_per_r); });
+    if (pos_div < threshold && a.prev_indicator[i] != 1)
+    {
+        bool near_previous_surface = false;
+        for (uint32_t k = 0; k < nc; ++k)
+            near_previous_surface |= a.prev_indicator[a.slice[nb + (k << 6)]] == 1;
+        if (!near_previous_surface)
+            pos_div = T(2) * threshold;
+    }
+    for (int k = 0; k < a.n_wall; ++k)
+    {
+        const WallArgs<T> &w = a.wall[k];
+        const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+        T wall_div = 0;
+        for (uint32_t k2 = 0; k2 < mc; ++k2)
+            wall_div -= term(load_pv<T, D>(w.posvol, w.slice[mb + (k2 << 6)]), poly_w[k], w.poly->x_per_r);
+        pos_div += wall_div;
+    }
+    a.pos_div[i] = pos_div;
+}
+
+// ... and the update: Indicator = 0 when the divergence exceeds the threshold and
+// no inner neighbour closer than h is below it, else 1; PreviousSurfaceIndicator = Indicator
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_surface_update(FluidArgs<typename Prec<P>::T> a)
+{
+    using T = typename Prec<P>::T;
+    const uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T threshold = T(0.75) * T(D);
+    int32_t indicator = 1;
+    if (a.pos_div[i] > threshold)
+    {
+        const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+        bool very_near_surface = false;
+        for_each_neighbor(a.slice, a.chunk[i >> 6] + (i & 63u), a.count[i], [&](uint32_t j) {
+            const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+            const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+            const T r = sqrt(D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy);
+            very_near_surface |= (a.pos_div[j] < threshold) & (r < a.h);
+        });
+        if (!very_near_surface)
+            indicator = 0;
+    }
+    a.indicator[i] = indicator;
+    a.prev_indicator[i] = indicator;
 }
 
 //---------------------------------------------------------------------------
@@ -866,6 +1191,13 @@ static FluidArgs<T> fluid_args(sphx_body *b)
     a.dC = (T *)b->dC;
     a.S = (T *)b->S;
     a.Vol_ref = (const T *)b->Vol_ref;
+    for (int c = 0; c < 9; ++c)
+        a.corr[c] = (T *)b->corr[c];
+    for (int k = 0; k < 3; ++k)
+        a.grad_sum[k] = (T *)b->grad_sum[k];
+    a.indicator = b->indicator;
+    a.prev_indicator = b->prev_indicator;
+    a.pos_div = (T *)b->pos_div;
     const double rho0 = b->desc.rho0, c0 = b->desc.c0;
     a.rho0 = T(rho0);
     a.c0 = T(c0);
@@ -973,7 +1305,11 @@ struct RefreshL
         return SPHX_OK;
     }
 };
-sphx_status refresh_posvol(sphx_body *b) { return dispatch<RefreshL>(b); }
+sphx_status refresh_posvol(sphx_body *b)
+{
+    b->state_version++;
+    return dispatch<RefreshL>(b);
+}
 
 template <int D, int P>
 struct GravityL
@@ -1024,7 +1360,11 @@ struct SetupL
         return SPHX_OK;
     }
 };
-sphx_status advection_step_setup(sphx_body *b) { return dispatch<SetupL>(b); }
+sphx_status advection_step_setup(sphx_body *b)
+{
+    b->state_version++;
+    return dispatch<SetupL>(b);
+}
 
 template <int D, int P>
 struct PositionL
@@ -1041,7 +1381,11 @@ struct PositionL
         return SPHX_OK;
     }
 };
-sphx_status update_position(sphx_body *b) { return dispatch<PositionL>(b); }
+sphx_status update_position(sphx_body *b)
+{
+    b->state_version++;
+    return dispatch<PositionL>(b);
+}
 
 template <int D, int P>
 struct SummationL
@@ -1067,11 +1411,69 @@ sphx_status compression_summation(sphx_relation *inner, sphx_relation *const *c,
     return dispatch<SummationL>(inner->source, inner, c, nc);
 }
 
+// launch one of the eight boolean instantiations of a pass kernel
+#define SPHX_PASS_VARIANTS(K, B0, B1, B2, ...)                                                                   \
+    do                                                                                                           \
+    {                                                                                                            \
+        if (B0 && B1 && B2)                                                                                      \
+            LAUNCH_PASS((K<D, P, true, true, true>), __VA_ARGS__);                                               \
+        else if (B0 && B1)                                                                                       \
+            LAUNCH_PASS((K<D, P, true, true, false>), __VA_ARGS__);                                              \
+        else if (B0 && B2)                                                                                       \
+            LAUNCH_PASS((K<D, P, true, false, true>), __VA_ARGS__);                                              \
+        else if (B0)                                                                                             \
+            LAUNCH_PASS((K<D, P, true, false, false>), __VA_ARGS__);                                             \
+        else if (B1 && B2)                                                                                       \
+            LAUNCH_PASS((K<D, P, false, true, true>), __VA_ARGS__);                                              \
+        else if (B1)                                                                                             \
+            LAUNCH_PASS((K<D, P, false, true, false>), __VA_ARGS__);                                             \
+        else if (B2)                                                                                             \
+            LAUNCH_PASS((K<D, P, false, false, true>), __VA_ARGS__);                                             \
+        else                                                                                                     \
+            LAUNCH_PASS((K<D, P, false, false, false>), __VA_ARGS__);                                            \
+    } while (0)
+#define LAUNCH_PASS(KERNEL, ...)                                                                                 \
+    do                                                                                                           \
+    {                                                                                                            \
+        allow_lds(KERNEL, lds);                                                                                  \
+        hipLaunchKernelGGL(KERNEL, g, blk, lds, b->ctx->stream, __VA_ARGS__);                                   \
+    } while (0)
+
+template <int D, int P>
+struct CorrectionGradL
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        attach_relations(a, inner, nullptr, 0);
+        hipLaunchKernelGGL((k_correction_grad<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
+                           b->ctx->stream, a);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+// grad_sum for the current positions, volumes, correction matrices and inner lists
+static sphx_status refresh_grad_sum(sphx_body *b, sphx_relation *inner)
+{
+    sphx_body::GradKey key{b->state_version, inner, inner->builds};
+    if (b->grad_key.state == key.state && b->grad_key.rel == key.rel && b->grad_key.builds == key.builds)
+        return SPHX_OK;
+    ProfileScope prof(b->ctx, "linear_correction");
+    sphx_status s = dispatch<CorrectionGradL>(b, inner);
+    if (!s)
+        b->grad_key = key;
+    return s;
+}
+
 template <int D, int P>
 struct Acoustic1L
 {
     static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
-                           double dt, bool fuse_init_2nd)
+                           double dt, bool fuse_init_2nd, bool corr)
     {
         using T = typename Prec<P>::T;
         if (!b->n)
@@ -1087,26 +1489,7 @@ struct Acoustic1L
         ProfileScope prof(b->ctx, "acoustic_pass1");
         const bool rs = riemann != SPHX_RIEMANN_NONE;
         const size_t lds = a.stage_count ? (size_t)a.stage_cap * (sizeof(Rec4<T>) + sizeof(T)) : 0;
-        if (rs && fuse_init_2nd)
-        {
-            allow_lds(k_pass1<D, P, true, true>, lds);
-            hipLaunchKernelGGL((k_pass1<D, P, true, true>), g, blk, lds, b->ctx->stream, a, dt);
-        }
-        else if (rs)
-        {
-            allow_lds(k_pass1<D, P, true, false>, lds);
-            hipLaunchKernelGGL((k_pass1<D, P, true, false>), g, blk, lds, b->ctx->stream, a, dt);
-        }
-        else if (fuse_init_2nd)
-        {
-            allow_lds(k_pass1<D, P, false, true>, lds);
-            hipLaunchKernelGGL((k_pass1<D, P, false, true>), g, blk, lds, b->ctx->stream, a, dt);
-        }
-        else
-        {
-            allow_lds(k_pass1<D, P, false, false>, lds);
-            hipLaunchKernelGGL((k_pass1<D, P, false, false>), g, blk, lds, b->ctx->stream, a, dt);
-        }
+        SPHX_PASS_VARIANTS(k_pass1, rs, fuse_init_2nd, corr, a, dt);
         LAUNCH_CHECK();
         return SPHX_OK;
     }
@@ -1116,7 +1499,7 @@ template <int D, int P>
 struct Acoustic2L
 {
     static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
-                           double dt, bool run_init, unsigned long long *max_slot)
+                           double dt, bool run_init, unsigned long long *max_slot, bool corr)
     {
         using T = typename Prec<P>::T;
         if (!b->n)
@@ -1132,27 +1515,9 @@ struct Acoustic2L
         }
         ProfileScope prof(b->ctx, "acoustic_pass2");
         const bool rs = riemann != SPHX_RIEMANN_NONE;
+        const bool cfl = max_slot != nullptr;
         const size_t lds = a.stage_count ? (size_t)a.stage_cap * 2 * sizeof(Rec4<T>) : 0;
-        if (rs && max_slot)
-        {
-            allow_lds(k_pass2<D, P, true, true>, lds);
-            hipLaunchKernelGGL((k_pass2<D, P, true, true>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
-        }
-        else if (rs)
-        {
-            allow_lds(k_pass2<D, P, true, false>, lds);
-            hipLaunchKernelGGL((k_pass2<D, P, true, false>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
-        }
-        else if (max_slot)
-        {
-            allow_lds(k_pass2<D, P, false, true>, lds);
-            hipLaunchKernelGGL((k_pass2<D, P, false, true>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
-        }
-        else
-        {
-            allow_lds(k_pass2<D, P, false, false>, lds);
-            hipLaunchKernelGGL((k_pass2<D, P, false, false>), g, blk, lds, b->ctx->stream, a, dt, max_slot);
-        }
+        SPHX_PASS_VARIANTS(k_pass2, rs, cfl, corr, a, dt, max_slot);
         LAUNCH_CHECK();
         return SPHX_OK;
     }
@@ -1161,7 +1526,7 @@ struct Acoustic2L
 sphx_status acoustic_1st_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann, double dt,
                               bool fuse_init_2nd)
 {
-    return dispatch<Acoustic1L>(inner->source, inner, w, nw, riemann, dt, fuse_init_2nd);
+    return dispatch<Acoustic1L>(inner->source, inner, w, nw, riemann, dt, fuse_init_2nd, false);
 }
 
 sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann, double dt,
@@ -1169,7 +1534,75 @@ sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int
 {
     sphx_body *b = inner->source;
     unsigned long long *slot = fuse_cfl ? b->ctx->scalars_d + SLOT_MAX : nullptr;
-    return dispatch<Acoustic2L>(b, inner, w, nw, riemann, dt, run_init, slot);
+    return dispatch<Acoustic2L>(b, inner, w, nw, riemann, dt, run_init, slot, false);
+}
+
+sphx_status acoustic_1st_half_corrected(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                                        double dt, bool fuse_init_2nd)
+{
+    sphx_status s = refresh_grad_sum(inner->source, inner);
+    if (s)
+        return s;
+    return dispatch<Acoustic1L>(inner->source, inner, w, nw, riemann, dt, fuse_init_2nd, true);
+}
+
+sphx_status acoustic_2nd_half_corrected(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                                        double dt, bool run_init, bool fuse_cfl)
+{
+    sphx_body *b = inner->source;
+    unsigned long long *slot = fuse_cfl ? b->ctx->scalars_d + SLOT_MAX : nullptr;
+    return dispatch<Acoustic2L>(b, inner, w, nw, riemann, dt, run_init, slot, true);
+}
+
+template <int D, int P>
+struct CorrectionL
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *c, int nc, double alpha)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        attach_relations(a, inner, c, nc);
+        const size_t lds = a.stage_count ? (size_t)a.stage_cap * sizeof(Rec4<T>) : 0;
+        allow_lds(k_correction<D, P>, lds);
+        hipLaunchKernelGGL((k_correction<D, P>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), lds,
+                           b->ctx->stream, a, alpha);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+sphx_status linear_correction_matrix(sphx_relation *inner, sphx_relation *const *c, int nc, double alpha)
+{
+    sphx_body *b = inner->source;
+    ProfileScope prof(b->ctx, "linear_correction");
+    b->state_version++;
+    return dispatch<CorrectionL>(b, inner, c, nc, alpha);
+}
+
+template <int D, int P>
+struct SurfaceL
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *c, int nc)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        attach_relations(a, inner, c, nc);
+        dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
+        hipLaunchKernelGGL((k_surface_div<D, P>), g, blk, 0, b->ctx->stream, a);
+        hipLaunchKernelGGL((k_surface_update<D, P>), g, blk, 0, b->ctx->stream, a);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+sphx_status free_surface_indication(sphx_relation *inner, sphx_relation *const *c, int nc)
+{
+    ProfileScope prof(inner->source->ctx, "surface_indication");
+    return dispatch<SurfaceL>(inner->source, inner, c, nc);
 }
 
 sphx_status fill_u64(sphx_context *ctx, unsigned long long *p, size_t n, unsigned long long v)

@@ -284,6 +284,35 @@ sphx_status transfer(sphx_body *b, int field, void *host, bool write)
         }
         return SPHX_OK;
     }
+    case SPHX_FIELD_CORRECTION_MATRIX:
+    {
+        // host: D x D row-major per particle; device: one array per component
+        std::vector<T> tmp(n);
+        for (int c = 0; c < D * D; ++c)
+        {
+            if (write)
+            {
+                for (size_t i = 0; i < n; ++i)
+                    tmp[i] = T(h[i * D * D + c]);
+                SPHX_HIP(hipMemcpyAsync(b->corr[c], tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+                SPHX_HIP(hipStreamSynchronize(s));
+            }
+            else
+            {
+                SPHX_HIP(hipMemcpyAsync(tmp.data(), b->corr[c], n * sizeof(T), hipMemcpyDeviceToHost, s));
+                SPHX_HIP(hipStreamSynchronize(s));
+                for (size_t i = 0; i < n; ++i)
+                    h[i * D * D + c] = (double)tmp[i];
+            }
+        }
+        return SPHX_OK;
+    }
+    case SPHX_FIELD_INDICATOR:
+        return u32_dev((uint32_t *)b->indicator);
+    case SPHX_FIELD_PREVIOUS_INDICATOR:
+        return u32_dev((uint32_t *)b->prev_indicator);
+    case SPHX_FIELD_POSITION_DIVERGENCE:
+        return scalar_dev(b->pos_div);
     case SPHX_FIELD_ORIGINAL_ID:
         return u32_dev(b->original_id);
     case SPHX_FIELD_SORTED_ID:
@@ -455,6 +484,18 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
         (s = alloc(b.get(), (void **)&b->original_id, n * 4)) || (s = alloc(b.get(), (void **)&b->sorted_id, n * 4)) ||
         (s = alloc(b.get(), (void **)&b->sequence, n * 4)) || (s = alloc(b.get(), (void **)&b->perm, n * 4)))
         return s;
+    // correction matrices, their neighbour sum and the surface indicators: fluid bodies only
+    const bool fluid = d->kind == SPHX_BODY_FLUID;
+    const size_t nf = fluid ? n : 0;
+    for (int c = 0; c < b->D * b->D; ++c)
+        if ((s = alloc(b.get(), &b->corr[c], nf * T)))
+            return s;
+    for (int k = 0; k < b->D; ++k)
+        if ((s = alloc(b.get(), &b->grad_sum[k], nf * T)))
+            return s;
+    if ((s = alloc(b.get(), (void **)&b->indicator, nf * 4)) ||
+        (s = alloc(b.get(), (void **)&b->prev_indicator, nf * 4)) || (s = alloc(b.get(), &b->pos_div, nf * T)))
+        return s;
     // zero every field, then the reference's initial values
     for (int k = 0; k < 3; ++k)
     {
@@ -486,6 +527,23 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
     }
     if ((s = fill_iota(ctx, b->original_id, n)) || (s = fill_iota(ctx, b->sorted_id, n)))
         return s;
+    // LinearCorrectionMatrix = identity (kernel_correction_ck.hpp:14-15); Indicator = 0,
+    // PositionDivergence = 0, PreviousSurfaceIndicator = 1 (surface_indication_ck.hpp:17-18, 41)
+    for (int c = 0; c < b->D * b->D; ++c)
+    {
+        const bool diag = c % (b->D + 1) == 0;
+        if ((s = b->P == P_FP64 ? fill<double>(ctx, b->corr[c], nf, diag ? 1.0 : 0.0)
+                                : fill<float>(ctx, b->corr[c], nf, diag ? 1.0f : 0.0f)))
+            return s;
+    }
+    if ((s = fill<int32_t>(ctx, b->indicator, nf, 0)) || (s = fill<int32_t>(ctx, b->prev_indicator, nf, 1)))
+        return s;
+    if (nf)
+    {
+        SPHX_HIP(hipMemsetAsync(b->pos_div, 0, nf * T, ctx->stream));
+        for (int k = 0; k < b->D; ++k)
+            SPHX_HIP(hipMemsetAsync(b->grad_sum[k], 0, nf * T, ctx->stream));
+    }
     SPHX_HIP(hipStreamSynchronize(ctx->stream));
     *out = b.release();
     return SPHX_OK;
@@ -518,12 +576,18 @@ static sphx_status body_rw(sphx_body *b, int field, void *host, size_t n, bool w
         return arg_error("particle count mismatch");
     if (field < 0 || field >= SPHX_FIELD_COUNT)
         return arg_error("unknown field id");
+    if ((field == SPHX_FIELD_CORRECTION_MATRIX || field == SPHX_FIELD_INDICATOR ||
+         field == SPHX_FIELD_PREVIOUS_INDICATOR || field == SPHX_FIELD_POSITION_DIVERGENCE) &&
+        b->desc.kind != SPHX_BODY_FLUID)
+        return arg_error("correction and surface-indication fields belong to fluid bodies");
     if (b->n == 0)
         return SPHX_OK;
     SPHX_HIP(hipSetDevice(b->ctx->device));
     sphx_status s = b->P == P_FP64 ? transfer<double>(b, field, host, write) : transfer<float>(b, field, host, write);
     if (s || !write)
         return s;
+    if (field == SPHX_FIELD_CORRECTION_MATRIX)
+        b->state_version++;
     if (field == SPHX_FIELD_VOLUME && !b->vol_written)
     {
         // first write: VolumetricMeasureRef = Vol (density_regularization.hpp:18-20),
@@ -834,6 +898,7 @@ sphx_status sphx_update_relation(sphx_relation *r)
         return s;
     r->csr_valid = false;
     r->built = true;
+    r->builds++;
     r->built_version = r->source->config_version;
     r->target_version = r->target->config_version;
     return SPHX_OK;
@@ -907,6 +972,9 @@ sphx_status sphx_particle_sort(sphx_body *b, sphx_cell_linked_list *c)
         (s = gather_real(b, &b->C, b->perm)) || (s = gather_real(b, &b->dC, b->perm)) ||
         (s = gather_real(b, &b->Vol_ref, b->perm)) || (s = sorted_id_update(b)) || (s = refresh_posvol(b)))
         return s;
+    // evolving once a free-surface indication exists (surface_indication_ck.hpp:43)
+    if (b->desc.kind == SPHX_BODY_FLUID && (s = gather_u32(b->ctx, (uint32_t *)b->prev_indicator, b->perm, b->n)))
+        return s;
     b->config_version++;
     return SPHX_OK;
 }
@@ -977,6 +1045,50 @@ sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *con
         return arg_error("unknown Riemann solver");
     SPHX_HIP(hipSetDevice(inner->source->ctx->device));
     return acoustic_2nd_half(inner, walls, n, riemann, dt, true, false);
+}
+
+sphx_status sphx_acoustic_step_1st_half_corrected(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                                  int riemann, double dt)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n, true);
+    if (s)
+        return s;
+    if (riemann < SPHX_RIEMANN_ACOUSTIC || riemann > SPHX_RIEMANN_DISSIPATIVE)
+        return arg_error("unknown Riemann solver");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return acoustic_1st_half_corrected(inner, walls, n, riemann, dt, false);
+}
+
+sphx_status sphx_acoustic_step_2nd_half_corrected(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                                  int riemann, double dt)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n, true);
+    if (s)
+        return s;
+    if (riemann < SPHX_RIEMANN_ACOUSTIC || riemann > SPHX_RIEMANN_DISSIPATIVE)
+        return arg_error("unknown Riemann solver");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return acoustic_2nd_half_corrected(inner, walls, n, riemann, dt, true, false);
+}
+
+sphx_status sphx_linear_correction_matrix(sphx_relation *inner, sphx_relation *const *contact, int n, double alpha)
+{
+    sphx_status s = check_fluid_relations(inner, contact, n, true);
+    if (s)
+        return s;
+    if (!std::isfinite(alpha))
+        return arg_error("alpha must be finite");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return linear_correction_matrix(inner, contact, n, alpha);
+}
+
+sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *const *contact, int n)
+{
+    sphx_status s = check_fluid_relations(inner, contact, n, true);
+    if (s)
+        return s;
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return free_surface_indication(inner, contact, n);
 }
 
 // the max lanes: reset before a max reduction, combined when read
@@ -1089,6 +1201,8 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
         return s;
     if (!cfg || !time || !cll || cll != inner->cll)
         return arg_error("advection step needs the fluid cell linked list, a config and a time pointer");
+    if (cfg->correction != SPHX_CORRECTION_NONE && cfg->correction != SPHX_CORRECTION_LINEAR)
+        return arg_error("unknown kernel correction");
     sphx_body *b = inner->source;
     sphx_context *ctx = b->ctx;
     SPHX_HIP(hipSetDevice(ctx->device));
@@ -1098,6 +1212,12 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     double adv_dt = 0.0;
     if ((s = sphx_advection_time_step(b, cfg->u_ref, cfg->advection_cfl, &adv_dt)))
         return s;
+    // dambreak.cpp:192-193
+    if (cfg->surface_indication && (s = free_surface_indication(inner, walls, n_wall)))
+        return s;
+    const bool corrected = cfg->correction == SPHX_CORRECTION_LINEAR;
+    if (corrected && (s = linear_correction_matrix(inner, walls, n_wall, cfg->correction_alpha)))
+        return s;
     // AcousticTimeStepCK of the first sub-step from the current state
     double dt = 0.0;
     if ((s = sphx_acoustic_time_step(b, cfg->acoustic_cfl, &dt)))
@@ -1106,9 +1226,17 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     uint32_t count = 0;
     while (relaxation_time < adv_dt)
     {
-        if ((s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, dt, true)) ||
-            (s = reset_max(ctx)) ||
-            (s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, dt, false, true)))
+        if (corrected)
+            s = acoustic_1st_half_corrected(inner, walls, n_wall, cfg->riemann, dt, true);
+        else
+            s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, dt, true);
+        if (s || (s = reset_max(ctx)))
+            return s;
+        if (corrected)
+            s = acoustic_2nd_half_corrected(inner, walls, n_wall, cfg->riemann, dt, false, true);
+        else
+            s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, dt, false, true);
+        if (s)
             return s;
         relaxation_time += dt;
         *time += dt;

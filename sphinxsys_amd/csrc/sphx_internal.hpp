@@ -90,6 +90,22 @@ struct sphx_body
     void *Vol = nullptr, *mass = nullptr, *rho = nullptr, *p = nullptr, *C = nullptr;
     void *dC = nullptr, *Vol_ref = nullptr, *S = nullptr, *interp = nullptr;
     uint32_t *original_id = nullptr, *sorted_id = nullptr, *sequence = nullptr, *perm = nullptr;
+    // "LinearCorrectionMatrix": component (a, b) in corr[a * D + b]; and, for the
+    // corrected pressure force, grad_sum = sum_j B_j e_ij dW_ij V_j over the inner
+    // neighbours, valid for the state stamped in grad_key
+    void *corr[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *grad_sum[3] = {nullptr, nullptr, nullptr};
+    struct GradKey
+    {
+        uint64_t state = ~0ull;
+        const sphx_relation *rel = nullptr;
+        uint64_t builds = 0;
+    } grad_key;
+    // bumped whenever positions, volumes or the correction matrices change
+    uint64_t state_version = 0;
+    // "Indicator", "PreviousSurfaceIndicator", "PositionDivergence"
+    int32_t *indicator = nullptr, *prev_indicator = nullptr;
+    void *pos_div = nullptr;
     std::vector<void *> allocations;
 };
 
@@ -136,6 +152,7 @@ struct sphx_relation
     uint16_t *local_slice = nullptr;
     size_t local_capacity = 0;
     bool built = false;
+    uint64_t builds = 0;                  // completed list builds
     uint64_t built_version = 0, target_version = 0; // config versions the lists were built for
     double inv_h = 0, src_inv_h = 0;      // Neighbor<SPHAdaptation,SPHAdaptation> constants
     void *poly = nullptr;                 // KernelPoly<T> on the device
@@ -177,6 +194,13 @@ sphx_status acoustic_1st_half(sphx_relation *inner, sphx_relation *const *w, int
 // fuse_cfl: max-reduce the next sub-step's signal speed into the SLOT_MAX lanes
 sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                               double dt, bool run_init, bool fuse_cfl);
+// LinearCorrectionCK variants of the two passes (the 1st refreshes grad_sum when stale)
+sphx_status acoustic_1st_half_corrected(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                                        double dt, bool fuse_init_2nd);
+sphx_status acoustic_2nd_half_corrected(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                                        double dt, bool run_init, bool fuse_cfl);
+sphx_status linear_correction_matrix(sphx_relation *inner, sphx_relation *const *c, int nc, double alpha);
+sphx_status free_surface_indication(sphx_relation *inner, sphx_relation *const *c, int nc);
 sphx_status acoustic_signal_max(sphx_body *b, unsigned long long *slot);
 sphx_status advection_speed_max(sphx_body *b, unsigned long long *slot);
 sphx_status acoustic_dt_from_max(sphx_body *b, double cfl, const unsigned long long *slot, double *dt_out);

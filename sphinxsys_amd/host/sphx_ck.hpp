@@ -83,7 +83,11 @@ class Error : public std::runtime_error
     X(sphx_interpolate_contact)                                                                           \
     X(sphx_advection_step)                                                                                \
     X(sphx_profile_enable)                                                                                \
-    X(sphx_profile_read)
+    X(sphx_profile_read)                                                                                  \
+    X(sphx_linear_correction_matrix)                                                                      \
+    X(sphx_acoustic_step_1st_half_corrected)                                                              \
+    X(sphx_acoustic_step_2nd_half_corrected)                                                              \
+    X(sphx_free_surface_indication)
 
 class Api
 {
@@ -244,6 +248,10 @@ inline int field_id(const std::string &name)
         {"SortedID", SPHX_FIELD_SORTED_ID},
         {"SortSequence", SPHX_FIELD_SORT_SEQUENCE},
         {"Interpolated", SPHX_FIELD_INTERPOLATED},
+        {"LinearCorrectionMatrix", SPHX_FIELD_CORRECTION_MATRIX},
+        {"Indicator", SPHX_FIELD_INDICATOR},
+        {"PreviousSurfaceIndicator", SPHX_FIELD_PREVIOUS_INDICATOR},
+        {"PositionDivergence", SPHX_FIELD_POSITION_DIVERGENCE},
     };
     auto it = ids.find(name);
     if (it == ids.end())
@@ -261,6 +269,8 @@ inline bool is_index_field(int id)
 {
     return id == SPHX_FIELD_ORIGINAL_ID || id == SPHX_FIELD_SORTED_ID || id == SPHX_FIELD_SORT_SEQUENCE;
 }
+// int32 fields (read_int / write of std::vector<int32_t>)
+inline bool is_int_field(int id) { return id == SPHX_FIELD_INDICATOR || id == SPHX_FIELD_PREVIOUS_INDICATOR; }
 
 // a body with its particles (FluidBody / SolidBody / ObserverBody)
 class SPHBody
@@ -314,12 +324,28 @@ class SPHBody
         expect(v.size(), id);
         api().check(api().sphx_body_write(h_, id, v.data(), n_));
     }
+    void write(const std::string &variable, const std::vector<int32_t> &v)
+    {
+        const int id = field_id(variable);
+        if (!is_int_field(id) || v.size() != n_)
+            throw Error(variable + ": expected " + std::to_string(n_) + " int32 values of an int field");
+        api().check(api().sphx_body_write(h_, id, v.data(), n_));
+    }
     std::vector<double> read(const std::string &variable) const
     {
         const int id = field_id(variable);
-        if (is_index_field(id))
-            throw Error(variable + " is an index field: use read_index");
-        std::vector<double> v(is_vector_field(id) ? n_ * p_.dim : n_);
+        if (is_index_field(id) || is_int_field(id))
+            throw Error(variable + " is an integer field: use read_index or read_int");
+        std::vector<double> v(values_per_particle(id) * n_);
+        api().check(api().sphx_body_read(h_, id, v.data(), n_));
+        return v;
+    }
+    std::vector<int32_t> read_int(const std::string &variable) const
+    {
+        const int id = field_id(variable);
+        if (!is_int_field(id))
+            throw Error(variable + " is not an int field");
+        std::vector<int32_t> v(n_);
         api().check(api().sphx_body_read(h_, id, v.data(), n_));
         return v;
     }
@@ -334,9 +360,15 @@ class SPHBody
     }
 
   private:
+    size_t values_per_particle(int id) const
+    {
+        if (id == SPHX_FIELD_CORRECTION_MATRIX)
+            return (size_t)p_.dim * p_.dim;
+        return is_vector_field(id) ? (size_t)p_.dim : 1;
+    }
     void expect(size_t got, int id) const
     {
-        const size_t want = is_vector_field(id) ? n_ * p_.dim : n_;
+        const size_t want = values_per_particle(id) * n_;
         if (got != want)
             throw Error("field size " + std::to_string(got) + ", expected " + std::to_string(want));
     }
@@ -560,7 +592,8 @@ class CompressionSummation : public FluidInteraction
     }
 };
 
-template <int RIEMANN>
+// CORRECTION: SPHX_CORRECTION_NONE (NoKernelCorrectionCK) or SPHX_CORRECTION_LINEAR (LinearCorrectionCK)
+template <int RIEMANN, int CORRECTION = SPHX_CORRECTION_NONE>
 class AcousticStep1stHalfCK : public FluidInteraction
 {
   public:
@@ -568,11 +601,13 @@ class AcousticStep1stHalfCK : public FluidInteraction
     AcousticStep1stHalfCK(Inner &inner, Contact &wall) : FluidInteraction(inner, {&wall}) {}
     void interact(double dt)
     {
-        api().check(api().sphx_acoustic_step_1st_half(inner_.handle(), walls(), (int)walls_.size(), RIEMANN, dt));
+        auto f = CORRECTION == SPHX_CORRECTION_LINEAR ? api().sphx_acoustic_step_1st_half_corrected
+                                                      : api().sphx_acoustic_step_1st_half;
+        api().check(f(inner_.handle(), walls(), (int)walls_.size(), RIEMANN, dt));
     }
 };
 
-template <int RIEMANN>
+template <int RIEMANN, int CORRECTION = SPHX_CORRECTION_NONE>
 class AcousticStep2ndHalfCK : public FluidInteraction
 {
   public:
@@ -580,15 +615,52 @@ class AcousticStep2ndHalfCK : public FluidInteraction
     AcousticStep2ndHalfCK(Inner &inner, Contact &wall) : FluidInteraction(inner, {&wall}) {}
     void interact(double dt)
     {
-        api().check(api().sphx_acoustic_step_2nd_half(inner_.handle(), walls(), (int)walls_.size(), RIEMANN, dt));
+        auto f = CORRECTION == SPHX_CORRECTION_LINEAR ? api().sphx_acoustic_step_2nd_half_corrected
+                                                      : api().sphx_acoustic_step_2nd_half;
+        api().check(f(inner_.handle(), walls(), (int)walls_.size(), RIEMANN, dt));
     }
 };
 
-// acoustic_step_1st_half.h:196-198 / acoustic_step_2nd_half.h:186-188
+// acoustic_step_1st_half.h:196-201 / acoustic_step_2nd_half.h:183-191
 using AcousticStep1stHalfWithWallRiemannCK = AcousticStep1stHalfCK<SPHX_RIEMANN_ACOUSTIC>;
 using AcousticStep2ndHalfWithWallRiemannCK = AcousticStep2ndHalfCK<SPHX_RIEMANN_ACOUSTIC>;
 using AcousticStep1stHalfWithWallNoRiemannCK = AcousticStep1stHalfCK<SPHX_RIEMANN_NONE>;
 using AcousticStep2ndHalfWithWallNoRiemannCK = AcousticStep2ndHalfCK<SPHX_RIEMANN_NONE>;
+using AcousticStep1stHalfWithWallRiemannCorrectionCK =
+    AcousticStep1stHalfCK<SPHX_RIEMANN_ACOUSTIC, SPHX_CORRECTION_LINEAR>;
+using AcousticStep2ndHalfWithWallRiemannCorrectionCK =
+    AcousticStep2ndHalfCK<SPHX_RIEMANN_ACOUSTIC, SPHX_CORRECTION_LINEAR>;
+
+// LinearCorrectionMatrix<Inner<WithUpdate>, Contact<>> (kernel_correction_ck.h:40-120), built as
+// LinearCorrectionMatrixComplex(DynamicsArgs(inner, alpha), wall)
+class LinearCorrectionMatrixComplex : public FluidInteraction
+{
+  public:
+    LinearCorrectionMatrixComplex(Inner &inner, double alpha = 0.0) : FluidInteraction(inner, {}), alpha_(alpha) {}
+    LinearCorrectionMatrixComplex(Inner &inner, double alpha, Contact &wall)
+        : FluidInteraction(inner, {&wall}), alpha_(alpha)
+    {
+    }
+    void interact()
+    {
+        api().check(api().sphx_linear_correction_matrix(inner_.handle(), walls(), (int)walls_.size(), alpha_));
+    }
+
+  private:
+    double alpha_;
+};
+
+// FreeSurfaceIndicationCK<Inner<WithUpdate>, Contact<>> (surface_indication_ck.h:170)
+class FreeSurfaceIndicationComplexSpatialTemporalCK : public FluidInteraction
+{
+  public:
+    FreeSurfaceIndicationComplexSpatialTemporalCK(Inner &inner) : FluidInteraction(inner, {}) {}
+    FreeSurfaceIndicationComplexSpatialTemporalCK(Inner &inner, Contact &wall) : FluidInteraction(inner, {&wall}) {}
+    void interact()
+    {
+        api().check(api().sphx_free_surface_indication(inner_.handle(), walls(), (int)walls_.size()));
+    }
+};
 
 // reduce dynamics (ReduceDynamicsCK<P, ...>)
 class AcousticTimeStepCK
@@ -731,12 +803,14 @@ class AdvectionStep
   public:
     AdvectionStep(Inner &inner, std::initializer_list<Contact *> walls, CellLinkedList &fluid_cll, double u_ref,
                   int riemann = SPHX_RIEMANN_ACOUSTIC, int flow_type = SPHX_FLOW_FREE_SURFACE,
-                  double advection_cfl = 0.25, double acoustic_cfl = 0.6)
+                  double advection_cfl = 0.25, double acoustic_cfl = 0.6, int correction = SPHX_CORRECTION_NONE,
+                  double correction_alpha = 0.5, bool surface_indication = false)
         : inner_(inner), cll_(fluid_cll)
     {
         for (Contact *w : walls)
             walls_.push_back(w->handle());
-        cfg_ = {u_ref, advection_cfl, acoustic_cfl, riemann, flow_type, 0};
+        cfg_ = {u_ref, advection_cfl, acoustic_cfl, riemann, flow_type, 0, correction, correction_alpha,
+                surface_indication ? 1 : 0};
     }
     uint32_t exec(double &physical_time, bool sort_now = false)
     {
