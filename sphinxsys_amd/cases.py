@@ -85,6 +85,29 @@ def dambreak_3d(dp=0.00448, DL=3.2, DW=1.0, DH=1.5, LL=1.2, LW=1.0, LH=0.6, wall
                             {"DL": DL, "DW": DW, "DH": DH, "LL": LL, "LW": LW, "LH": LH, "BW": BW})
 
 
+def dambreak_3d_ck(dp=0.05, DL=5.366, DH=2.0, DW=0.5, LL=2.0, LH=1.0, LW=0.5, wall_layers=4,
+                   rho0=1.0, g=1.0, order="morton"):
+    """The CK 3D dam break of tests/tests_sycl/3d_examples/test_3d_dambreak_sycl/dambreak.cpp:10-65:
+    x length, y height (gravity along -y), z width; BW = 4 dp; six pressure probes
+    on the end wall at x = DL, z = DW / 2."""
+    BW = dp * wall_layers
+    U_ref = 2.0 * math.sqrt(g * LH)
+    c0 = 10.0 * U_ref
+    half_w = (0.5 * LL, 0.5 * LH, 0.5 * LW)
+    water = Box(half_w, half_w)
+    half_in = (0.5 * DL, 0.5 * DH, 0.5 * DW)
+    half_out = (0.5 * DL + BW, 0.5 * DH + BW, 0.5 * DW + BW)
+    wall = ComplexShape().add(Box(half_in, half_out)).subtract(Box(half_in, half_in))
+    lo, up = system_bounds((-BW, -BW, -BW), (DL + BW, DH + BW, DW + BW), dp)
+    fluid = _ordered(lattice_positions(lo, up, dp, water), lo, dp, order)
+    wall_pos = _ordered(lattice_positions(lo, up, dp, wall), lo, dp, order)
+    normals = normal_from_body_shape(wall, wall_pos)
+    probes = np.array([[DL, y, 0.5 * DW] for y in (0.01, 0.1, 0.2, 0.24, 0.252, 0.266)])
+    return DamBreakGeometry(3, dp, fluid, wall_pos, normals, lo, up, probes, rho0, c0,
+                            np.array([0.0, -g, 0.0]), U_ref,
+                            {"DL": DL, "DW": DW, "DH": DH, "LL": LL, "LW": LW, "LH": LH, "BW": BW})
+
+
 def dp_for_fluid_count(n_target, LL=1.2, LW=1.0, LH=0.6):
     """Particle spacing giving about `n_target` fluid particles in the 3D column."""
     return (LL * LW * LH / float(n_target)) ** (1.0 / 3.0)
@@ -204,6 +227,25 @@ class DamBreak:
     def observe(self):
         self.observer_relation.exec()
         return self.probe.exec(), self.energy.exec()
+
+    def run_ck_loop(self, end_time, output_interval):
+        """The main loop of the CK 3D dam break (dambreak.cpp:183-232): advection
+        steps until each output interval has elapsed, the probe pressures after
+        every advection step (and before the first) and the mechanical energy
+        after every output interval (and before the first).  Returns
+        (pressure[steps + 1, probes], energy[intervals + 1])."""
+        pressure = [self.observe()[0]]
+        energy = [self.energy.exec()]
+        while self.time < end_time:
+            integration_time = 0.0
+            while integration_time < output_interval:
+                t0 = self.time
+                self.advection_step()
+                integration_time += self.time - t0
+                self.observer_relation.exec()
+                pressure.append(self.probe.exec())
+            energy.append(self.energy.exec())
+        return np.array(pressure), np.array(energy)
 
     def close(self):
         for o in (self.observer_contact, self.contact, self.inner, self.wall_cll, self.fluid_cll,

@@ -56,3 +56,24 @@ def dtw_distance(a, b, window=5):
         for j in range(max(1, i - window), min(lb, i + window)):
             d[i, j] = abs(a[i] - b[j]) + min(d[i - 1, j], d[i, j - 1], d[i - 1, j - 1])
     return d[la - 1, lb - 1]
+
+
+def dtw_distance_banded(a, b, window=5):
+    """The same recurrence with the cells outside the band at +infinity, so that
+    the end cell is always reached through the band.  The reference's table
+    starts at zero everywhere, which makes its distance 0 whenever the second
+    series is longer than the first by the band width or more."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    la, lb = len(a), len(b)
+    window = max(window, abs(la - lb) + 1)
+    d = np.full((la, lb), np.inf)
+    d[0, 0] = abs(a[0] - b[0])
+    for i in range(1, la):
+        d[i, 0] = d[i - 1, 0] + abs(a[i] - b[0])
+    for j in range(1, lb):
+        d[0, j] = d[0, j - 1] + abs(a[0] - b[j])
+    for i in range(1, la):
+        lo, hi = max(1, i - window), min(lb, i + window)
+        for j in range(lo, hi):
+            d[i, j] = abs(a[i] - b[j]) + min(d[i - 1, j], d[i, j - 1], d[i - 1, j - 1])
+    return d[la - 1, lb - 1]
