@@ -289,7 +289,8 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
  * "relation_build" (tiled list build), "relation_scan" (per-lane scan build,
  * used when a list outgrows the LDS tile), "relation_stage" (LDS staging sets
  * of the FP32-mixed inner relation), "cell_linked_list", "particle_sort",
- * "time_step_reduce", "state_update".  Reading synchronises the stream and
+ * "time_step_reduce", "state_update", "linear_correction" (correction matrix and
+ * its neighbour sum), "surface_indication".  Reading synchronises the stream and
  * returns the accumulated milliseconds and launch count since enabling. */
 sphx_status sphx_profile_enable(sphx_context *ctx, int enable);
 sphx_status sphx_profile_read(sphx_context *ctx, const char *stage, double *total_ms,
