@@ -228,23 +228,42 @@ class DamBreak:
         self.observer_relation.exec()
         return self.probe.exec(), self.energy.exec()
 
-    def run_ck_loop(self, end_time, output_interval):
-        """The main loop of the CK 3D dam break (dambreak.cpp:183-232): advection
+    def run_ck_loop(self, end_time, output_interval, pressure_recording=None, energy_recording=None):
+        """The main loop of the CK 3D dam break (dambreak.cpp:170-232): advection
         steps until each output interval has elapsed, the probe pressures after
         every advection step (and before the first) and the mechanical energy
-        after every output interval (and before the first).  Returns
-        (pressure[steps + 1, probes], energy[intervals + 1])."""
-        pressure = [self.observe()[0]]
-        energy = [self.energy.exec()]
+        after every output interval (and before the first).  The recordings
+        (objects with write_to_file(physical_time, iteration), e.g. the
+        regression wrappers of sphinxsys_amd.recording) receive each write with
+        the iteration count, as fluid_observer_pressure / record_water_mechanical_energy
+        do.  Returns (pressure[steps + 1, probes], energy[intervals + 1])."""
+        pressure, energy = [], []
+
+        def observe_pressure():
+            self.observer_relation.exec()
+            if pressure_recording is not None:
+                pressure_recording.write_to_file(self.time, self.iterations)
+                pressure.append(np.array(pressure_recording.observed()))
+            else:
+                pressure.append(self.probe.exec())
+
+        def observe_energy():
+            if energy_recording is not None:
+                energy_recording.write_to_file(self.time, self.iterations)
+                energy.append(float(energy_recording.observed()[0]))
+            else:
+                energy.append(self.energy.exec())
+
+        observe_energy()
+        observe_pressure()
         while self.time < end_time:
             integration_time = 0.0
             while integration_time < output_interval:
                 t0 = self.time
                 self.advection_step()
                 integration_time += self.time - t0
-                self.observer_relation.exec()
-                pressure.append(self.probe.exec())
-            energy.append(self.energy.exec())
+                observe_pressure()
+            observe_energy()
         return np.array(pressure), np.array(energy)
 
     def close(self):

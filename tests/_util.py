@@ -1,10 +1,10 @@
-"""Test helpers: oracle loading, reference fixture parsing, DTW distance."""
+"""Test helpers: oracle loading, fixture paths, a banded DTW distance."""
 import os
 import subprocess
-import xml.etree.ElementTree as ET
 
 import numpy as np
 
+from sphinxsys_amd import recording
 from sphinxsys_amd._capi import SphxLibrary
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -22,40 +22,11 @@ def oracle_lib():
     return lib
 
 
-def read_result_xml(path):
-    """Particle_k snapshot_s values of a regression_test_tool *_result.xml."""
-    root = ET.parse(path).getroot()
-    series = []
-    for el in root.find("Result_Element"):
-        n = len(el.attrib)
-        series.append(np.array([float(el.attrib[f"snapshot_{s}"]) for s in range(n)]))
-    return series
-
-
-def read_dtw_threshold(path):
-    el = ET.parse(path).getroot().find("DTWDistance")
-    return [float(v) for v in el.attrib.values()]
-
-
-def dtw_distance(a, b, window=5):
-    """RegressionTestDynamicTimeWarping::calculateDTWDistance
-    (src/shared/regression_test/dynamic_time_warping_method.hpp:16-55), p-norm of
-    scalars = absolute difference."""
-    a, b = np.asarray(a, float), np.asarray(b, float)
-    la, lb = len(a), len(b)
-    if lb > 1.2 * la or lb < 0.8 * la:
-        raise ValueError("series lengths differ by more than 20%")
-    d = np.zeros((la, lb))
-    d[0, 0] = abs(a[0] - b[0])
-    for i in range(1, la):
-        d[i, 0] = d[i - 1, 0] + abs(a[i] - b[0])
-    for j in range(1, lb):
-        d[0, j] = d[0, j - 1] + abs(a[0] - b[j])
-    window = max(window, abs(la - lb))
-    for i in range(1, la):
-        for j in range(max(1, i - window), min(lb, i + window)):
-            d[i, j] = abs(a[i] - b[j]) + min(d[i - 1, j], d[i, j - 1], d[i - 1, j - 1])
-    return d[la - 1, lb - 1]
+# the regression files and the reference's DTW rule live in the package
+# (sphinxsys_amd.recording); the tests use the same functions
+read_result_xml = recording.read_result_xml
+read_dtw_threshold = recording.read_dtw_xml
+dtw_distance = recording.dtw_distance
 
 
 def dtw_distance_banded(a, b, window=5):
