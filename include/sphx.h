@@ -75,7 +75,10 @@ enum {
     SPHX_FIELD_INDICATOR = 21,        /* "Indicator"               int32 (1: free surface) */
     SPHX_FIELD_PREVIOUS_INDICATOR = 22,/* "PreviousSurfaceIndicator" int32, initial 1   */
     SPHX_FIELD_POSITION_DIVERGENCE = 23,/* "PositionDivergence"    Real  */
-    SPHX_FIELD_COUNT = 24
+    SPHX_FIELD_VISCOUS_FORCE = 24,    /* "ViscousForce"          Vecd  */
+    SPHX_FIELD_PREVIOUS_VISCOUS_FORCE = 25, /* "PreviousViscousForce" Vecd (evolving) */
+    SPHX_FIELD_KERNEL_GRADIENT_INTEGRAL = 26, /* "KernelGradientIntegral" Vecd */
+    SPHX_FIELD_COUNT = 27
 };
 
 /* Riemann solvers (reference: riemann_solver_ck.h:166-172,
@@ -92,6 +95,10 @@ enum { SPHX_FLOW_INTERNAL = 0, SPHX_FLOW_FREE_SURFACE = 1 };
 /* Kernel corrections of the acoustic half steps (reference: NoKernelCorrectionCK,
  * LinearCorrectionCK, general_dynamics/kernel_correction_ck.h:153-187). */
 enum { SPHX_CORRECTION_NONE = 0, SPHX_CORRECTION_LINEAR = 1 };
+
+/* Limiters of the transport-velocity correction (reference: NoLimiter, TruncatedLinear
+ * with its default slope 100, common/common_functors.h:69-94). */
+enum { SPHX_LIMITER_NONE = 0, SPHX_LIMITER_TRUNCATED_LINEAR = 1 };
 
 typedef struct sphx_context sphx_context;
 typedef struct sphx_body sphx_body;
@@ -230,6 +237,27 @@ sphx_status sphx_acoustic_step_2nd_half_corrected(sphx_relation *inner, sphx_rel
 sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *const *contact,
                                          int n_contact);
 
+/* reference: InteractionDynamicsCK<P, ViscousForceCK<Inner<WithUpdate, Viscosity, K>,
+ * Contact<Wall, Viscosity, K>>> (fluid_dynamics/viscous_force.hpp:38-104) with the update
+ * of ForcePriorCK (general_dynamics/force_prior_ck.h:50-59): "ViscousForce" from the inner
+ * and wall neighbours for the constant viscosity mu (Viscosity material), then
+ * ForcePrior += ViscousForce - PreviousViscousForce; PreviousViscousForce = ViscousForce.
+ * correction: SPHX_CORRECTION_* (K; ViscousForceWithWallCK uses NONE). */
+sphx_status sphx_viscous_force(sphx_relation *inner, sphx_relation *const *walls, int n_wall, double mu,
+                               int correction);
+/* reference: InteractionDynamicsCK<P, KernelGradientIntegral<Inner<K>, Contact<Boundary, K>>>
+ * (general_dynamics/kernel_gradient_integral.hpp:39-100; KernelGradientIntegralComplex /
+ * ...CorrectedComplex): "KernelGradientIntegral" = -sum (B_i + B_j) dW V_j e_ij over the inner
+ * neighbours - sum 2 B_i dW V_j e_ij over the contact neighbours. */
+sphx_status sphx_kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *contact,
+                                          int n_contact, int correction);
+/* reference: StateDynamics<P, TransportVelocityCorrectionCK<SPHBody, Limiter[, BulkParticles]>>
+ * (fluid_dynamics/transport_velocity_correction_ck.hpp:11-51): Displacement += coefficient h^2
+ * limiter(h^2 |g|^2) g for g = KernelGradientIntegral (uniform resolution: h ratio 1); with
+ * bulk_only, only particles whose Indicator is 0. coefficient 0.2 by default in the reference. */
+sphx_status sphx_transport_velocity_correction(sphx_body *fluid, double coefficient, int limiter,
+                                               int bulk_only);
+
 /* ---- reduce dynamics -------------------------------------------------------- */
 /* reference: ReduceDynamicsCK<P, AcousticTimeStepCK<WeaklyCompressibleFluid>>
  * (fluid_dynamics/fluid_time_step_ck.hpp:30-57); acoustic_cfl default 0.6. */
@@ -246,9 +274,9 @@ sphx_status sphx_upper_front_in_axis(sphx_body *fluid, int axis, double *front);
 
 /* ---- observation -------------------------------------------------------------- */
 /* reference: Interpolation<Contact<Real>> over an observer contact relation
- * (general_dynamics/interpolation_dynamics.hpp:46-60); field is a scalar field
- * of the contact body (e.g. SPHX_FIELD_PRESSURE).  Result per observer point
- * in out[n_observer]. */
+ * (general_dynamics/interpolation_dynamics.hpp:46-60); field is SPHX_FIELD_PRESSURE,
+ * _DENSITY or _COMPRESSION of the contact body (result out[n_observer]) or
+ * SPHX_FIELD_VELOCITY (Interpolation<Contact<Vecd>>, result out[n_observer * dim]). */
 sphx_status sphx_interpolate_contact(sphx_relation *observer_contact, int field, double *out,
                                      size_t n_observer);
 

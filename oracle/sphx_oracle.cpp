@@ -66,6 +66,7 @@ struct Body : BodyBase
     std::vector<UInt> original_id, sorted_id, sequence;
     std::vector<Mat<R, D>> correction;         // "LinearCorrectionMatrix", initial identity
     std::vector<int32_t> indicator, prev_indicator; // "Indicator" (0), "PreviousSurfaceIndicator" (1)
+    std::vector<V> viscous_force, prev_viscous_force, kernel_gradient_integral;
 
     explicit Body(const sphx_body_desc &d)
     {
@@ -85,6 +86,7 @@ struct Body : BodyBase
         // kernel_correction_ck.hpp:12-15; surface_indication_ck.hpp:17-18, 41-44
         correction.assign(n, Mat<R, D>::Identity());
         indicator.assign(n, 0), prev_indicator.assign(n, 1);
+        viscous_force.assign(n, z), prev_viscous_force.assign(n, z), kernel_gradient_integral.assign(n, z);
         for (UInt i = 0; i < n; ++i)
             original_id[i] = sorted_id[i] = i;
     }
@@ -557,6 +559,105 @@ static sphx_status surface_indication(sphx_relation *inner, sphx_relation *const
     return SPHX_OK;
 }
 
+// ViscousForceCK<Inner<WithUpdate, Viscosity, K>> + <Contact<Wall, Viscosity, K>>
+// (viscous_force.hpp:14-104) and the ForcePriorCK update (force_prior_ck.h:55-59)
+template <class R, int D>
+static sphx_status viscous_force(sphx_relation *inner, sphx_relation *const *walls, int n_wall, double mu_d,
+                                 bool corrected)
+{
+    Body<R, D> &b = as<R, D>(inner->source);
+    const R mu = R(mu_d);
+    // Viscosity::getInterParticleViscosity = PairGeomAverageFixed(mu, mu), particle_functors.h:169-170
+    const R mu_pair = R(2.0) * mu * mu * (R(1) / (mu + mu));
+    const R h_sq = std::pow(b.h, R(2)); // pow(ReferenceSmoothingLength(), 2), viscous_force.hpp:24
+    NeighborKernel<R, D> nk(*b.kernel, b.h, b.h);
+    parallel_for(b.n, [&](size_t i) {
+        Vec<R, D> force = Vec<R, D>::Zero();
+        for (UInt n = inner->particle_offset[i]; n != inner->particle_offset[i + 1]; ++n)
+        {
+            UInt j = inner->neighbor_index[n];
+            Vec<R, D> r = b.pos[i] - b.pos[j];
+            Vec<R, D> e_ij = normalized(r);
+            R dW_ijV_j = nk.dW(r) * b.Vol[j];
+            Vec<R, D> vel_derivative = (b.vel[i] - b.vel[j]) / (squaredNorm(r) + R(0.01) * h_sq);
+            Vec<R, D> ce = corrected ? (b.correction[i] + b.correction[j]) * e_ij : (R(1.0) + R(1.0)) * e_ij;
+            force += dot(r, ce) * mu_pair * vel_derivative * dW_ijV_j;
+        }
+        b.viscous_force[i] = force * b.Vol[i];
+    });
+    for (int k = 0; k < n_wall; ++k)
+    {
+        sphx_relation *w = walls[k];
+        Body<R, D> &wb = as<R, D>(w->target);
+        NeighborKernel<R, D> wk(*b.kernel, b.h, wb.h);
+        parallel_for(b.n, [&](size_t i) {
+            Vec<R, D> force = Vec<R, D>::Zero();
+            for (UInt n = w->particle_offset[i]; n != w->particle_offset[i + 1]; ++n)
+            {
+                UInt j = w->neighbor_index[n];
+                Vec<R, D> r = b.pos[i] - wb.pos[j];
+                Vec<R, D> e_ij = normalized(r);
+                R dW_ijV_j = wk.dW(r) * wb.Vol[j];
+                Vec<R, D> vel_derivative = R(2.0) * (b.vel[i] - wb.wall_vel[j]) / (squaredNorm(r) + R(0.01) * h_sq);
+                Vec<R, D> ce = corrected ? b.correction[i] * e_ij : R(1.0) * e_ij;
+                force += R(2.0) * dot(r, ce) * mu * vel_derivative * dW_ijV_j;
+            }
+            b.viscous_force[i] += force * b.Vol[i];
+        });
+    }
+    parallel_for(b.n, [&](size_t i) {
+        b.force_prior[i] += b.viscous_force[i] - b.prev_viscous_force[i];
+        b.prev_viscous_force[i] = b.viscous_force[i];
+    });
+    return SPHX_OK;
+}
+
+// KernelGradientIntegral<Inner<K>> + <Contact<Boundary, K>> (kernel_gradient_integral.hpp:39-100)
+template <class R, int D>
+static sphx_status kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *contact, int n_contact,
+                                            bool corrected)
+{
+    Body<R, D> &b = as<R, D>(inner->source);
+    NeighborKernel<R, D> nk(*b.kernel, b.h, b.h);
+    parallel_for(b.n, [&](size_t i) {
+        Vec<R, D> inconsistency = Vec<R, D>::Zero();
+        for (UInt n = inner->particle_offset[i]; n != inner->particle_offset[i + 1]; ++n)
+        {
+            UInt j = inner->neighbor_index[n];
+            Vec<R, D> r = b.pos[i] - b.pos[j];
+            const R dW_ijV_j = nk.dW(r) * b.Vol[j];
+            const Vec<R, D> e_ij = normalized(r);
+            if (corrected)
+                inconsistency -= (b.correction[i] + b.correction[j]) * dW_ijV_j * e_ij;
+            else
+                inconsistency -= (R(1.0) + R(1.0)) * dW_ijV_j * e_ij;
+        }
+        b.kernel_gradient_integral[i] = inconsistency;
+    });
+    for (int k = 0; k < n_contact; ++k)
+    {
+        sphx_relation *w = contact[k];
+        Body<R, D> &wb = as<R, D>(w->target);
+        NeighborKernel<R, D> wk(*b.kernel, b.h, wb.h);
+        parallel_for(b.n, [&](size_t i) {
+            Vec<R, D> inconsistency = Vec<R, D>::Zero();
+            for (UInt n = w->particle_offset[i]; n != w->particle_offset[i + 1]; ++n)
+            {
+                UInt j = w->neighbor_index[n];
+                Vec<R, D> r = b.pos[i] - wb.pos[j];
+                const R dW_ijV_j = wk.dW(r) * wb.Vol[j];
+                const Vec<R, D> e_ij = normalized(r);
+                if (corrected)
+                    inconsistency -= R(2.0) * b.correction[i] * dW_ijV_j * e_ij;
+                else
+                    inconsistency -= R(2.0) * R(1.0) * dW_ijV_j * e_ij;
+            }
+            b.kernel_gradient_integral[i] += inconsistency;
+        });
+    }
+    return SPHX_OK;
+}
+
 // fixed-block summation so that the result does not depend on the thread count
 template <class F>
 static double block_sum(size_t n, const F &f)
@@ -756,6 +857,9 @@ static sphx_status body_rw(sphx_body *b, int field, void *host, size_t n, bool w
         case SPHX_FIELD_INDICATOR: return rw_int(B.indicator, host, n, write);
         case SPHX_FIELD_PREVIOUS_INDICATOR: return rw_int(B.prev_indicator, host, n, write);
         case SPHX_FIELD_POSITION_DIVERGENCE: return rw_scalar(B.pos_div, host, n, write);
+        case SPHX_FIELD_VISCOUS_FORCE: return rw_vec(B.viscous_force, host, n, write);
+        case SPHX_FIELD_PREVIOUS_VISCOUS_FORCE: return rw_vec(B.prev_viscous_force, host, n, write);
+        case SPHX_FIELD_KERNEL_GRADIENT_INTEGRAL: return rw_vec(B.kernel_gradient_integral, host, n, write);
         default: return fail(SPHX_E_ARG, "unknown field");
         }
     });
@@ -934,6 +1038,8 @@ sphx_status sphx_particle_sort(sphx_body *body, sphx_cell_linked_list *cll)
         gather(B.force_prior), gather(B.C), gather(B.dC), gather(B.Vol_ref);
         // evolving once the free-surface indication exists (surface_indication_ck.hpp:43)
         gather(B.prev_indicator);
+        // evolving with a viscous force (force_prior_ck.cpp:14)
+        gather(B.prev_viscous_force);
         for (UInt i = 0; i < B.n; ++i)
             B.sorted_id[B.original_id[i]] = i;
         return SPHX_OK;
@@ -1058,6 +1164,47 @@ sphx_status sphx_linear_correction_matrix(sphx_relation *inner, sphx_relation *c
         return correction_matrix<R, D>(inner, contact, n, alpha);
     });
 }
+sphx_status sphx_viscous_force(sphx_relation *inner, sphx_relation *const *walls, int n, double mu, int correction)
+{
+    if (!relations_ok(inner, walls, n) || !(mu >= 0.0))
+        return fail(SPHX_E_ARG, "viscous force needs updated relations of one source and mu >= 0");
+    return with_body(inner->source, [&](auto &B) {
+        BODY_TYPES(B);
+        return viscous_force<R, D>(inner, walls, n, mu, correction == SPHX_CORRECTION_LINEAR);
+    });
+}
+sphx_status sphx_kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *contact, int n, int correction)
+{
+    if (!relations_ok(inner, contact, n))
+        return fail(SPHX_E_ARG, "kernel gradient integral needs updated relations of one source");
+    return with_body(inner->source, [&](auto &B) {
+        BODY_TYPES(B);
+        return kernel_gradient_integral<R, D>(inner, contact, n, correction == SPHX_CORRECTION_LINEAR);
+    });
+}
+// TransportVelocityCorrectionCK::UpdateKernel::update, transport_velocity_correction_ck.hpp:39-51;
+// TruncatedLinear(100), common_functors.h:82-94; BulkParticles: Indicator == 0
+sphx_status sphx_transport_velocity_correction(sphx_body *body, double coefficient, int limiter, int bulk_only)
+{
+    return with_body(body, [&](auto &B) {
+        BODY_TYPES(B);
+        const R h_ref = B.h;
+        const R squared_h_ref = h_ref * h_ref;
+        const R correction_scaling = R(coefficient) * squared_h_ref;
+        parallel_for(B.n, [&](size_t i) {
+            if (bulk_only && B.indicator[i] != 0)
+                return;
+            const R inv_h_ratio = R(1.0) / R(1.0);
+            const Vec<R, D> &residual = B.kernel_gradient_integral[i];
+            const R squared_norm = squaredNorm(residual);
+            const R lim = limiter == SPHX_LIMITER_TRUNCATED_LINEAR
+                              ? std::min(R(100.0) * (squared_h_ref * squared_norm), R(1))
+                              : R(1.0);
+            B.dpos[i] += correction_scaling * lim * B.kernel_gradient_integral[i] * inv_h_ratio * inv_h_ratio;
+        });
+        return SPHX_OK;
+    });
+}
 sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *const *contact, int n)
 {
     if (!relations_ok(inner, contact, n))
@@ -1138,6 +1285,27 @@ sphx_status sphx_interpolate_contact(sphx_relation *rel, int field, double *out,
         if (n_obs != O.n)
             return fail(SPHX_E_ARG, "observer count mismatch");
         const std::vector<R> *data = nullptr;
+        NeighborKernel<R, D> nk(*O.kernel, O.h, T.h);
+        if (field == SPHX_FIELD_VELOCITY)
+        {
+            // Interpolation<Contact<Vecd>>: weighted sum of the vector / (total weight + TinyReal)
+            for (UInt i = 0; i < O.n; ++i)
+            {
+                Vec<R, D> q = Vec<R, D>::Zero();
+                R w(0);
+                for (UInt n = rel->particle_offset[i]; n != rel->particle_offset[i + 1]; ++n)
+                {
+                    UInt j = rel->neighbor_index[n];
+                    R weight_j = nk.W(O.pos[i] - T.pos[j]) * T.Vol[j];
+                    q += weight_j * T.vel[j];
+                    w += weight_j;
+                }
+                Vec<R, D> v = q / (w + TinyReal<R>());
+                for (int k = 0; k < D; ++k)
+                    out[i * D + k] = (double)v[k];
+            }
+            return SPHX_OK;
+        }
         if (field == SPHX_FIELD_PRESSURE)
             data = &T.p;
         else if (field == SPHX_FIELD_DENSITY)
@@ -1146,7 +1314,6 @@ sphx_status sphx_interpolate_contact(sphx_relation *rel, int field, double *out,
             data = &T.C;
         else
             return fail(SPHX_E_ARG, "unsupported interpolation field");
-        NeighborKernel<R, D> nk(*O.kernel, O.h, T.h);
         for (UInt i = 0; i < O.n; ++i)
         {
             R q(0), w(0);

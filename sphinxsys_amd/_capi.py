@@ -14,6 +14,7 @@ BODY_FLUID, BODY_SOLID, BODY_OBSERVER = 0, 1, 2
 RIEMANN_ACOUSTIC, RIEMANN_NONE, RIEMANN_DISSIPATIVE = 0, 1, 2
 FLOW_INTERNAL, FLOW_FREE_SURFACE = 0, 1
 CORRECTION_NONE, CORRECTION_LINEAR = 0, 1
+LIMITER_NONE, LIMITER_TRUNCATED_LINEAR = 0, 1
 
 FIELDS = {
     "Position": (0, "vec"), "VolumetricMeasure": (1, "real"), "Mass": (2, "real"),
@@ -26,6 +27,7 @@ FIELDS = {
     "SortSequence": (18, "uint"), "Interpolated": (19, "real"),
     "LinearCorrectionMatrix": (20, "mat"), "Indicator": (21, "int"),
     "PreviousSurfaceIndicator": (22, "int"), "PositionDivergence": (23, "real"),
+    "ViscousForce": (24, "vec"), "PreviousViscousForce": (25, "vec"), "KernelGradientIntegral": (26, "vec"),
 }
 
 # every symbol include/sphx.h declares
@@ -44,7 +46,8 @@ SYMBOLS = [
     "sphx_interpolate_contact", "sphx_advection_step", "sphx_profile_enable",
     "sphx_profile_read", "sphx_linear_correction_matrix",
     "sphx_acoustic_step_1st_half_corrected", "sphx_acoustic_step_2nd_half_corrected",
-    "sphx_free_surface_indication",
+    "sphx_free_surface_indication", "sphx_viscous_force", "sphx_kernel_gradient_integral",
+    "sphx_transport_velocity_correction",
 ]
 
 
@@ -120,6 +123,9 @@ class SphxLibrary:
             "sphx_acoustic_step_1st_half_corrected": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
             "sphx_acoustic_step_2nd_half_corrected": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
             "sphx_free_surface_indication": (C.c_int, [_vp, _pp, C.c_int]),
+            "sphx_viscous_force": (C.c_int, [_vp, _pp, C.c_int, C.c_double, C.c_int]),
+            "sphx_kernel_gradient_integral": (C.c_int, [_vp, _pp, C.c_int, C.c_int]),
+            "sphx_transport_velocity_correction": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -3,6 +3,9 @@
 2D: tests/2d_examples/test_2d_dambreak/Dambreak.cpp and
     tests/tests_sycl/2d_examples/test_2d_dambreak_sycl/dambreak_sycl.cpp
     (tank 5.366 x 5.366, column 2 x 1, dp 0.025, BW 4 dp, rho0 1, g 1).
+Lid-driven cavity: tests/tests_sycl/2d_examples/test_2d_lid_driven_cavity_corrected_sycl/
+    lid_driven_cavity_sycl.cpp (viscous force, kernel gradient integral and
+    transport-velocity correction on the corrected acoustic step).
 3D: the CK case tests/tests_sycl/3d_examples/test_3d_dambreak_sycl/dambreak.cpp
     with the tank/column/material parameters of the 8M-particle workload
     (tank 3.2 x 1.0 x 1.5 with z up, column 1.2 x 1.0 x 0.6 at x = 0,
@@ -269,6 +272,171 @@ class DamBreak:
     def close(self):
         for o in (self.observer_contact, self.contact, self.inner, self.wall_cll, self.fluid_cll,
                   self.observer, self.wall, self.fluid):
+            o.close()
+        if self.own_ctx:
+            self.ctx.close()
+
+
+# --------------------------------------------------------------------------
+# lid-driven cavity
+# --------------------------------------------------------------------------
+@dataclass
+class CavityGeometry:
+    dp: float
+    fluid_pos: np.ndarray
+    wall_pos: np.ndarray
+    wall_normal: np.ndarray
+    wall_vel: np.ndarray
+    bounds_lower: np.ndarray
+    bounds_upper: np.ndarray
+    horizontal_pos: np.ndarray
+    vertical_pos: np.ndarray
+    rho0: float
+    c0: float
+    mu: float
+    u_ref: float
+
+
+def lid_driven_cavity(dp=1.0 / 50.0, DL=1.0, DH=1.0, wall_layers=6, rho0=1.0, U=1.0, Re=100.0,
+                      order="morton"):
+    """lid_driven_cavity_sycl.cpp:9-117: unit square of fluid in a wall ring of 6 dp,
+    the wall above y = DH moving at (1, 0) (BoundaryVelocity), c = 10 U,
+    mu = rho0 U DL / Re, five velocity probes on each mid line."""
+    BW = dp * wall_layers
+    water = Box((0.5 * DL, 0.5 * DH), (0.5 * DL, 0.5 * DH))
+    wall = ComplexShape().add(Box((0.5 * DL, 0.5 * DH), (0.5 * DL + BW, 0.5 * DH + BW))) \
+                         .subtract(Box((0.5 * DL, 0.5 * DH), (0.5 * DL, 0.5 * DH)))
+    lo, up = system_bounds((-BW, -BW), (DL + BW, DH + BW), dp)
+    fluid = _ordered(lattice_positions(lo, up, dp, water), lo, dp, order)
+    wall_pos = _ordered(lattice_positions(lo, up, dp, wall), lo, dp, order)
+    normals = normal_from_body_shape(wall, wall_pos)
+    wall_vel = np.zeros_like(wall_pos)
+    wall_vel[wall_pos[:, 1] > DH, 0] = 1.0
+    rng_ = 1.0 - 0.5 * dp
+    start = 0.5 * dp
+    s = np.array([rng_ * i / 4.0 + start for i in range(5)])
+    horizontal = np.stack([s, np.full(5, 0.5 * DL)], axis=1)
+    vertical = np.stack([np.full(5, 0.5 * DH), s], axis=1)
+    return CavityGeometry(dp, fluid, wall_pos, normals, wall_vel, lo, up, horizontal, vertical, rho0,
+                          10.0 * U, rho0 * U * DL / Re, U)
+
+
+class LidDrivenCavity:
+    """Bodies, relations and the CK loop of the corrected lid-driven cavity
+    (lid_driven_cavity_sycl.cpp:181-330) on one library."""
+
+    def __init__(self, lib, geom, precision=capi.FP64, ctx=None):
+        self.lib, self.geom, self.precision = lib, geom, precision
+        self.own_ctx = ctx is None
+        self.ctx = ctx if ctx is not None else ck.Context(lib)
+        self.fluid = ck.Body(self.ctx, 2, len(geom.fluid_pos), geom.dp, capi.BODY_FLUID, precision,
+                             rho0=geom.rho0, c0=geom.c0)
+        self.wall = ck.Body(self.ctx, 2, len(geom.wall_pos), geom.dp, capi.BODY_SOLID, precision,
+                            rho0=geom.rho0, c0=geom.c0)
+        self.h_obs = ck.Body(self.ctx, 2, 5, geom.dp, capi.BODY_OBSERVER, precision, rho0=geom.rho0, c0=geom.c0)
+        self.v_obs = ck.Body(self.ctx, 2, 5, geom.dp, capi.BODY_OBSERVER, precision, rho0=geom.rho0, c0=geom.c0)
+        vol = geom.dp ** 2
+        for b, pos in ((self.fluid, geom.fluid_pos), (self.wall, geom.wall_pos), (self.h_obs, geom.horizontal_pos),
+                       (self.v_obs, geom.vertical_pos)):
+            b.write("Position", pos)
+            b.write("VolumetricMeasure", np.full(len(pos), vol))
+        self.wall.write("NormalDirection", geom.wall_normal)
+        lo, up = geom.bounds_lower, geom.bounds_upper
+        self.fluid_cll = ck.CellLinkedList(self.fluid, lo, up)
+        self.wall_cll = ck.CellLinkedList(self.wall, lo, up)
+        self.inner = ck.Inner(self.fluid, self.fluid_cll)
+        self.contact = ck.Contact(self.fluid, self.wall, self.wall_cll)
+        self.h_contact = ck.Contact(self.h_obs, self.fluid, self.fluid_cll)
+        self.v_contact = ck.Contact(self.v_obs, self.fluid, self.fluid_cll)
+        self.fluid_cell_linked_list = ck.UpdateCellLinkedList(self.fluid_cll)
+        self.wall_cell_linked_list = ck.UpdateCellLinkedList(self.wall_cll)
+        self.complex_relation = ck.UpdateRelation(self.inner, self.contact)
+        self.observer_relations = ck.UpdateRelation(self.h_contact, self.v_contact)
+        self.particle_sort = ck.ParticleSortCK(self.fluid, self.fluid_cll)
+        self.setup = ck.AdvectionStepSetup(self.fluid)
+        self.position = ck.UpdateParticlePosition(self.fluid)
+        self.correction_matrix = ck.LinearCorrectionMatrixComplex(self.inner, self.contact, alpha=0.5)
+        self.first_half = ck.AcousticStep1stHalfWithWallRiemannCorrectionCK(self.inner, self.contact)
+        self.second_half = ck.AcousticStep2ndHalfWithWallRiemannCK(self.inner, self.contact)
+        self.summation = ck.CompressionSummation(self.inner, self.contact)
+        self.regularization = ck.DensityRegularization(self.fluid, capi.FLOW_INTERNAL)
+        self.boundary_indicator = ck.FreeSurfaceIndicationComplexSpatialTemporalCK(self.inner, self.contact)
+        self.kernel_gradient_integral = ck.KernelGradientIntegralCorrectedComplex(self.inner, self.contact)
+        self.transport_correction = ck.TransportVelocityCorrectionCK(self.fluid, 0.2,
+                                                                     capi.LIMITER_TRUNCATED_LINEAR)
+        self.advection_dt = ck.AdvectionTimeStepCK(self.fluid, geom.u_ref)
+        self.acoustic_dt = ck.AcousticTimeStepCK(self.fluid)
+        self.viscous_force = ck.ViscousForceWithWallCK(self.inner, self.contact, mu=geom.mu)
+        self.h_velocity = ck.InterpolationContact(self.h_contact, "Velocity")
+        self.v_velocity = ck.InterpolationContact(self.v_contact, "Velocity")
+        self.time = 0.0
+        self.integration_time = 0.0
+        self.iterations = 0
+        self.acoustic_steps = 0
+
+    def initialize(self):
+        """lid_driven_cavity_sycl.cpp:241-249 (the wall normals come with the geometry)."""
+        self.fluid_cell_linked_list.exec()
+        self.wall_cell_linked_list.exec()
+        self.complex_relation.exec()
+        self.observer_relations.exec()
+        self.wall.write("WallVelocity", self.geom.wall_vel)  # solid_initial_condition
+        self.correction_matrix.exec()
+
+    def advection_step(self):
+        """One pass of the inner loop body, lid_driven_cavity_sycl.cpp:279-321."""
+        self.summation.exec()
+        self.regularization.exec()
+        self.setup.exec()
+        self.viscous_force.exec()
+        self.correction_matrix.exec()
+        self.boundary_indicator.exec()
+        self.kernel_gradient_integral.exec()
+        self.transport_correction.exec()
+        adv_dt = self.advection_dt.exec()
+        relaxation = 0.0
+        while relaxation < adv_dt:
+            dt = min(self.acoustic_dt.exec(), adv_dt)
+            self.first_half.exec(dt)
+            self.second_half.exec(dt)
+            relaxation += dt
+            self.integration_time += dt
+            self.time += dt
+            self.acoustic_steps += 1
+        self.position.exec()
+        self.iterations += 1
+        if self.iterations % 100 == 0 and self.iterations != 1:
+            self.particle_sort.exec()
+        self.fluid_cell_linked_list.exec()
+        self.complex_relation.exec()
+        return relaxation
+
+    def run(self, end_time, output_interval, horizontal_recording=None, vertical_recording=None):
+        """The outer loop (lid_driven_cavity_sycl.cpp:270-337): observer velocities
+        before the loop and after every output interval."""
+        h, v = [], []
+
+        def observe():
+            self.observer_relations.exec()
+            for rec, interp, out in ((horizontal_recording, self.h_velocity, h),
+                                     (vertical_recording, self.v_velocity, v)):
+                if rec is not None:
+                    rec.write_to_file(self.time, self.iterations)
+                    out.append(np.array(rec.observed()))
+                else:
+                    out.append(interp.exec())
+
+        observe()
+        while self.time < end_time:
+            self.integration_time = 0.0
+            while self.integration_time < output_interval:
+                self.advection_step()
+            observe()
+        return np.array(h), np.array(v)
+
+    def close(self):
+        for o in (self.h_contact, self.v_contact, self.contact, self.inner, self.wall_cll, self.fluid_cll,
+                  self.h_obs, self.v_obs, self.wall, self.fluid):
             o.close()
         if self.own_ctx:
             self.ctx.close()

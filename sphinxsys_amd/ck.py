@@ -14,6 +14,9 @@ Class names, constructor arguments and exec() semantics follow the reference:
   Interpolation<Contact<Real>>                  general_dynamics/interpolation_dynamics.hpp:46-60
   LinearCorrectionMatrix                        general_dynamics/kernel_correction_ck.h:40-120
   FreeSurfaceIndicationCK                       general_dynamics/surface_indication/surface_indication_ck.h
+  ViscousForceCK                                fluid_dynamics/viscous_force.h:43-127
+  KernelGradientIntegral                        general_dynamics/kernel_gradient_integral.h:36-96
+  TransportVelocityCorrectionCK                 fluid_dynamics/transport_velocity_correction_ck.h
 Every object takes the SphxLibrary it runs on; the product binding is
 `sphinxsys_amd.load()` (HIP).  Errors raise SphxError with the library's message.
 """
@@ -305,6 +308,54 @@ class FreeSurfaceIndicationComplexSpatialTemporalCK(_FluidInteraction):
         self.lib.call("sphx_free_surface_indication", self.inner.h, self._ptrs, len(self.contacts))
 
 
+class ViscousForceCK(_FluidInteraction):
+    """ViscousForceCK<Inner<WithUpdate, Viscosity, K>, Contact<Wall, Viscosity, K>> for the
+    body's constant viscosity `mu`; adds the change of "ViscousForce" to "ForcePrior"."""
+
+    def __init__(self, inner, *walls, mu, correction=capi.CORRECTION_NONE):
+        super().__init__(inner, *walls)
+        self.mu, self.correction = mu, correction
+
+    def exec(self, dt=0.0):
+        self.lib.call("sphx_viscous_force", self.inner.h, self._ptrs, len(self.contacts), float(self.mu),
+                      self.correction)
+
+
+def ViscousForceWithWallCK(inner, *walls, mu):
+    return ViscousForceCK(inner, *walls, mu=mu, correction=capi.CORRECTION_NONE)
+
+
+class KernelGradientIntegral(_FluidInteraction):
+    """KernelGradientIntegral<Inner<K>, Contact<Boundary, K>>: "KernelGradientIntegral"."""
+
+    def __init__(self, inner, *contacts, correction=capi.CORRECTION_NONE):
+        super().__init__(inner, *contacts)
+        self.correction = correction
+
+    def exec(self, dt=0.0):
+        self.lib.call("sphx_kernel_gradient_integral", self.inner.h, self._ptrs, len(self.contacts),
+                      self.correction)
+
+
+def KernelGradientIntegralComplex(inner, *contacts):
+    return KernelGradientIntegral(inner, *contacts, correction=capi.CORRECTION_NONE)
+
+
+def KernelGradientIntegralCorrectedComplex(inner, *contacts):
+    return KernelGradientIntegral(inner, *contacts, correction=capi.CORRECTION_LINEAR)
+
+
+class TransportVelocityCorrectionCK:
+    """TransportVelocityCorrectionCK<SPHBody, Limiter[, BulkParticles]>(body, coefficient)."""
+
+    def __init__(self, body, coefficient=0.2, limiter=capi.LIMITER_NONE, bulk_only=False):
+        self.body, self.coefficient, self.limiter, self.bulk_only = body, coefficient, limiter, bulk_only
+
+    def exec(self, dt=0.0):
+        self.body.lib.call("sphx_transport_velocity_correction", self.body.h, float(self.coefficient),
+                           self.limiter, 1 if self.bulk_only else 0)
+
+
 class AcousticTimeStepCK:
     def __init__(self, body, acoustic_cfl=0.6):
         self.body, self.cfl = body, acoustic_cfl
@@ -348,15 +399,20 @@ class UpperFrontInAxisDirectionCK:
 
 
 class InterpolationContact:
-    """Interpolation<Contact<Real>> of a scalar field of the contact body at observers."""
+    """Interpolation<Contact<Real>> of a scalar field, or Interpolation<Contact<Vecd>> of
+    "Velocity" (one row of dim values per observer), of the contact body at observers."""
 
     def __init__(self, observer_contact, field="Pressure"):
         self.rel = observer_contact
-        self.fid = capi.FIELDS[field][0]
+        self.fid, self.kind = capi.FIELDS[field]
 
     def exec(self, dt=0.0):
-        out = np.empty(self.rel.source.n, dtype=np.float64)
-        self.rel.lib.call("sphx_interpolate_contact", self.rel.h, self.fid, as_dp(out), out.size)
+        n = self.rel.source.n
+        if self.kind == "vec":
+            out = np.empty((n, self.rel.source.dim), dtype=np.float64)
+        else:
+            out = np.empty(n, dtype=np.float64)
+        self.rel.lib.call("sphx_interpolate_contact", self.rel.h, self.fid, as_dp(out), n)
         return out
 
 

@@ -67,7 +67,7 @@ struct FluidArgs
     const Rec4<T> *posvol;
     Rec4<T> *vel;
     T *force[3];
-    const T *force_prior[3];
+    T *force_prior[3];
     const T *mass;
     T *rho, *p, *C, *dC, *S;
     const T *Vol_ref;
@@ -91,6 +91,8 @@ struct FluidArgs
     // free-surface indication state
     int32_t *indicator, *prev_indicator;
     T *pos_div;
+    // viscous force, its previous value in ForcePrior, kernel gradient integral
+    T *visc[3], *prev_visc[3], *kgi[3];
 };
 
 // position/volume record of particle j
@@ -895,6 +897,221 @@ __global__ void __launch_bounds__(kThreads) k_surface_update(FluidArgs<typename 
 }
 
 //---------------------------------------------------------------------------
+// ViscousForceCK<Inner<WithUpdate, Viscosity, K>> + <Contact<Wall, Viscosity, K>>
+// (viscous_force.hpp:38-104) and the ForcePriorCK update (force_prior_ck.h:55-59):
+// F_i = V_i [ sum_j mu_ij (r.(C_i + C_j) e) (v_i - v_j) / (r^2 + 0.01 h^2) dW V_j
+//           + sum_w 2 mu (r.C_i e) 2 (v_i - v_w) / (r^2 + 0.01 h^2) dW V_w ],
+// C = 1 (CORR false) or the correction matrix; ForcePrior += F - F_previous.
+//---------------------------------------------------------------------------
+template <int D, int P, bool CORR>
+__global__ void __launch_bounds__(kThreads) k_viscous(FluidArgs<typename Prec<P>::T> a, double mu_pair_d,
+                                                      double mu_d, double h_sq_d)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    stage_poly(poly_in, a.poly, true);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, true);
+    __syncthreads();
+    const uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const T mu_pair = T(mu_pair_d), mu = T(mu_d), eps = T(0.01) * T(h_sq_d);
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    const Rec4<T> vi = a.vel[i];
+    Mat3<D, T> Bi;
+    if (CORR)
+        Bi.load(a.corr, i);
+    T fx = 0, fy = 0, fz = 0;
+    const T x_per_r = a.poly->x_per_r;
+    for_each_neighbor(a.slice, a.chunk[i >> 6] + (i & 63u), a.count[i], [&](uint32_t j) {
+        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+        const Rec4<T> vj = a.vel[j];
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T ir = inv_sqrt(r2);
+        const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
+        T s; // r . (C_i + C_j) e
+        if (CORR)
+        {
+            Mat3<D, T> Bj;
+            Bj.load(a.corr, j);
+            T bx, by, bz, cx, cy, cz;
+            const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+            Bi.apply(ex, ey, ez, bx, by, bz);
+            Bj.apply(ex, ey, ez, cx, cy, cz);
+            s = dx * (bx + cx) + dy * (by + cy);
+            if (D == 3)
+                s += dz * (bz + cz);
+        }
+        else
+            s = T(2) * (r2 * ir);
+        const T c = s * mu_pair * dWV / (r2 + eps);
+        fx += c * (vi.x - vj.x);
+        fy += c * (vi.y - vj.y);
+        if (D == 3)
+            fz += c * (vi.z - vj.z);
+    });
+    const T vol_i = pi.vol;
+    T Fx = fx * vol_i, Fy = fy * vol_i, Fz = fz * vol_i;
+    for (int k = 0; k < a.n_wall; ++k)
+    {
+        const WallArgs<T> &w = a.wall[k];
+        const T w_x_per_r = w.poly->x_per_r;
+        T wx = 0, wy = 0, wz = 0;
+        const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+        for (uint32_t k2 = 0; k2 < mc; ++k2)
+        {
+            const uint32_t j = w.slice[mb + (k2 << 6)];
+            const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
+            const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+            const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+            const T ir = inv_sqrt(r2);
+            const T dWV = eval_poly(poly_w[k], r2 * ir, w_x_per_r) * pj.vol;
+            T s; // r . C_i e
+            if (CORR)
+            {
+                T bx, by, bz;
+                Bi.apply(dx * ir, dy * ir, D == 3 ? dz * ir : T(0), bx, by, bz);
+                s = dx * bx + dy * by;
+                if (D == 3)
+                    s += dz * bz;
+            }
+            else
+                s = r2 * ir;
+            const T c = T(2) * s * mu * dWV / (r2 + eps);
+            wx += c * (T(2) * (vi.x - w.vel[0][j]));
+            wy += c * (T(2) * (vi.y - w.vel[1][j]));
+            if (D == 3)
+                wz += c * (T(2) * (vi.z - w.vel[2][j]));
+        }
+        Fx += wx * vol_i;
+        Fy += wy * vol_i;
+        if (D == 3)
+            Fz += wz * vol_i;
+    }
+    // ForcePriorCK::UpdateKernel::update
+    a.visc[0][i] = Fx;
+    a.visc[1][i] = Fy;
+    a.force_prior[0][i] += Fx - a.prev_visc[0][i];
+    a.force_prior[1][i] += Fy - a.prev_visc[1][i];
+    a.prev_visc[0][i] = Fx;
+    a.prev_visc[1][i] = Fy;
+    if (D == 3)
+    {
+        a.visc[2][i] = Fz;
+        a.force_prior[2][i] += Fz - a.prev_visc[2][i];
+        a.prev_visc[2][i] = Fz;
+    }
+}
+
+//---------------------------------------------------------------------------
+// KernelGradientIntegral<Inner<K>> + <Contact<Boundary, K>> (kernel_gradient_integral.hpp:57-100):
+// g_i = -sum_j (C_i + C_j) dW V_j e_ij - sum_w 2 C_i dW V_w e_iw
+//---------------------------------------------------------------------------
+template <int D, int P, bool CORR>
+__global__ void __launch_bounds__(kThreads) k_kernel_gradient_integral(FluidArgs<typename Prec<P>::T> a)
+{
+    using T = typename Prec<P>::T;
+    __shared__ PolyLDS<T> poly_in;
+    __shared__ PolyLDS<T> poly_w[kMaxWalls];
+    stage_poly(poly_in, a.poly, true);
+    for (int k = 0; k < a.n_wall; ++k)
+        stage_poly(poly_w[k], a.wall[k].poly, true);
+    __syncthreads();
+    const uint32_t i = xcd_block_index() * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const PV<T, D> pi = load_pv<T, D>(a.posvol, i);
+    Mat3<D, T> Bi;
+    if (CORR)
+        Bi.load(a.corr, i);
+    T gx = 0, gy = 0, gz = 0;
+    const T x_per_r = a.poly->x_per_r;
+    for_each_neighbor(a.slice, a.chunk[i >> 6] + (i & 63u), a.count[i], [&](uint32_t j) {
+        const PV<T, D> pj = load_pv<T, D>(a.posvol, j);
+        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+        const T ir = inv_sqrt(r2);
+        const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
+        const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+        if (CORR)
+        {
+            Mat3<D, T> Bj;
+            Bj.load(a.corr, j);
+            T bx, by, bz, cx, cy, cz;
+            Bi.apply(ex, ey, ez, bx, by, bz);
+            Bj.apply(ex, ey, ez, cx, cy, cz);
+            gx -= (bx + cx) * dWV;
+            gy -= (by + cy) * dWV;
+            if (D == 3)
+                gz -= (bz + cz) * dWV;
+        }
+        else
+        {
+            const T s = T(2) * dWV;
+            gx -= s * ex;
+            gy -= s * ey;
+            if (D == 3)
+                gz -= s * ez;
+        }
+    });
+    for (int k = 0; k < a.n_wall; ++k)
+    {
+        const WallArgs<T> &w = a.wall[k];
+        const T w_x_per_r = w.poly->x_per_r;
+        T wx = 0, wy = 0, wz = 0;
+        const uint32_t mb = w.chunk[i >> 6] + (i & 63u), mc = w.count[i];
+        for (uint32_t k2 = 0; k2 < mc; ++k2)
+        {
+            const uint32_t j = w.slice[mb + (k2 << 6)];
+            const PV<T, D> pj = load_pv<T, D>(w.posvol, j);
+            const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
+            const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+            const T ir = inv_sqrt(r2);
+            const T s = T(2) * eval_poly(poly_w[k], r2 * ir, w_x_per_r) * pj.vol;
+            T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+            if (CORR)
+                Bi.apply(dx * ir, dy * ir, D == 3 ? dz * ir : T(0), ex, ey, ez);
+            wx -= s * ex;
+            wy -= s * ey;
+            if (D == 3)
+                wz -= s * ez;
+        }
+        gx += wx;
+        gy += wy;
+        if (D == 3)
+            gz += wz;
+    }
+    a.kgi[0][i] = gx;
+    a.kgi[1][i] = gy;
+    if (D == 3)
+        a.kgi[2][i] = gz;
+}
+
+// TransportVelocityCorrectionCK::UpdateKernel::update (transport_velocity_correction_ck.hpp:39-51):
+// dpos += coefficient h^2 limiter(h^2 |g|^2) g (h ratio 1), TruncatedLinear(100) or no limiter,
+// all particles or those with Indicator 0
+template <int D, int P>
+__global__ void k_transport_correction(FluidArgs<typename Prec<P>::T> a, double scaling_d, double sq_h_d,
+                                       int truncated, int bulk_only)
+{
+    using T = typename Prec<P>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || (bulk_only && a.indicator[i] != 0))
+        return;
+    const T gx = a.kgi[0][i], gy = a.kgi[1][i], gz = D == 3 ? a.kgi[2][i] : T(0);
+    const T sq = D == 3 ? gx * gx + gy * gy + gz * gz : gx * gx + gy * gy;
+    const T lim = truncated ? dmin(T(100) * (T(sq_h_d) * sq), T(1)) : T(1);
+    const T c = T(scaling_d) * lim;
+    a.dpos[0][i] += (double)(c * gx);
+    a.dpos[1][i] += (double)(c * gy);
+    if (D == 3)
+        a.dpos[2][i] += (double)(c * gz);
+}
+
+//---------------------------------------------------------------------------
 // state dynamics
 //---------------------------------------------------------------------------
 template <int D, int P>
@@ -1139,7 +1356,7 @@ __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v
 // Interpolation<Contact<Real>>::interact (interpolation_dynamics.hpp:46-60)
 template <int D, class T>
 __global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> *tar, const T *data,
-                              const uint32_t *chunk, const uint32_t *count, const uint32_t *slice,
+                              uint32_t stride, const uint32_t *chunk, const uint32_t *count, const uint32_t *slice,
                               const KernelPoly<T> *poly, T *out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1160,7 +1377,7 @@ __global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> 
         const T *c = poly->w[loc];
         const T W = ((c[3] * f + c[2]) * f + c[1]) * f + c[0];
         const T weight = W * pj.vol;
-        q += weight * data[j];
+        q += weight * data[(size_t)j * stride];
         w += weight;
     }
     out[i] = q / (w + T(2.71051e-20));
@@ -1181,7 +1398,7 @@ static FluidArgs<T> fluid_args(sphx_body *b)
     for (int k = 0; k < 3; ++k)
     {
         a.force[k] = (T *)b->force[k];
-        a.force_prior[k] = (const T *)b->force_prior[k];
+        a.force_prior[k] = (T *)b->force_prior[k];
         a.dpos[k] = b->dpos[k];
     }
     a.mass = (const T *)b->mass;
@@ -1198,6 +1415,12 @@ static FluidArgs<T> fluid_args(sphx_body *b)
     a.indicator = b->indicator;
     a.prev_indicator = b->prev_indicator;
     a.pos_div = (T *)b->pos_div;
+    for (int k = 0; k < 3; ++k)
+    {
+        a.visc[k] = (T *)b->visc[k];
+        a.prev_visc[k] = (T *)b->prev_visc[k];
+        a.kgi[k] = (T *)b->kgi[k];
+    }
     const double rho0 = b->desc.rho0, c0 = b->desc.c0;
     a.rho0 = T(rho0);
     a.c0 = T(c0);
@@ -1582,6 +1805,85 @@ sphx_status linear_correction_matrix(sphx_relation *inner, sphx_relation *const 
 }
 
 template <int D, int P>
+struct ViscousL
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *w, int nw, double mu,
+                           bool corr)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        attach_relations(a, inner, w, nw);
+        // Viscosity::getInterParticleViscosity: PairGeomAverageFixed(mu, mu), particle_functors.h:169-170
+        const double mu_pair = 2.0 * mu * mu * (1.0 / (mu + mu));
+        const double h_sq = std::pow(b->h, 2.0);
+        dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
+        if (corr)
+            hipLaunchKernelGGL((k_viscous<D, P, true>), g, blk, 0, b->ctx->stream, a, mu_pair, mu, h_sq);
+        else
+            hipLaunchKernelGGL((k_viscous<D, P, false>), g, blk, 0, b->ctx->stream, a, mu_pair, mu, h_sq);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+sphx_status viscous_force(sphx_relation *inner, sphx_relation *const *w, int nw, double mu, bool corrected)
+{
+    ProfileScope prof(inner->source->ctx, "viscous_force");
+    return dispatch<ViscousL>(inner->source, inner, w, nw, mu, corrected);
+}
+
+template <int D, int P>
+struct KgiL
+{
+    static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *c, int nc, bool corr)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        FluidArgs<T> a = fluid_args<T>(b);
+        attach_relations(a, inner, c, nc);
+        dim3 g(grid_for(b->n, kThreads)), blk(kThreads);
+        if (corr)
+            hipLaunchKernelGGL((k_kernel_gradient_integral<D, P, true>), g, blk, 0, b->ctx->stream, a);
+        else
+            hipLaunchKernelGGL((k_kernel_gradient_integral<D, P, false>), g, blk, 0, b->ctx->stream, a);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+sphx_status kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *c, int nc, bool corrected)
+{
+    ProfileScope prof(inner->source->ctx, "transport_correction");
+    return dispatch<KgiL>(inner->source, inner, c, nc, corrected);
+}
+
+template <int D, int P>
+struct TransportL
+{
+    static sphx_status run(sphx_body *b, double coefficient, int limiter, bool bulk_only)
+    {
+        using T = typename Prec<P>::T;
+        if (!b->n)
+            return SPHX_OK;
+        const double sq_h = b->h * b->h;
+        hipLaunchKernelGGL((k_transport_correction<D, P>), dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream,
+                           fluid_args<T>(b), coefficient * sq_h, sq_h,
+                           limiter == SPHX_LIMITER_TRUNCATED_LINEAR ? 1 : 0, bulk_only ? 1 : 0);
+        LAUNCH_CHECK();
+        return SPHX_OK;
+    }
+};
+
+sphx_status transport_velocity_correction(sphx_body *b, double coefficient, int limiter, bool bulk_only)
+{
+    ProfileScope prof(b->ctx, "transport_correction");
+    return dispatch<TransportL>(b, coefficient, limiter, bulk_only);
+}
+
+template <int D, int P>
 struct SurfaceL
 {
     static sphx_status run(sphx_body *b, sphx_relation *inner, sphx_relation *const *c, int nc)
@@ -1680,23 +1982,30 @@ sphx_status mechanical_energy(sphx_body *b, const double *g, double *out) { retu
 template <int D, int P>
 struct InterpL
 {
-    static sphx_status run(sphx_body *o, sphx_relation *rel, int field)
+    static sphx_status run(sphx_body *o, sphx_relation *rel, int field, int component)
     {
         using T = typename Prec<P>::T;
         if (!o->n)
             return SPHX_OK;
         sphx_body *t = rel->target;
-        const T *data = field == SPHX_FIELD_PRESSURE ? (const T *)t->p
-                        : field == SPHX_FIELD_DENSITY ? (const T *)t->rho
-                                                      : (const T *)t->C;
+        // a velocity component k is every 4th value of the (x, y, z, .) records from k on
+        const T *data = field == SPHX_FIELD_PRESSURE   ? (const T *)t->p
+                        : field == SPHX_FIELD_DENSITY  ? (const T *)t->rho
+                        : field == SPHX_FIELD_VELOCITY ? (const T *)t->vel + component
+                                                       : (const T *)t->C;
+        const uint32_t stride = field == SPHX_FIELD_VELOCITY ? 4u : 1u;
         hipLaunchKernelGGL((k_interpolate<D, T>), dim3(grid_for(o->n, 64)), dim3(64), 0, o->ctx->stream,
-                           (const Rec4<T> *)o->posvol, o->n, (const Rec4<T> *)t->posvol, data, rel->chunk_offset,
-                           rel->neighbor_size, rel->slice, (const KernelPoly<T> *)rel->poly, (T *)o->interp);
+                           (const Rec4<T> *)o->posvol, o->n, (const Rec4<T> *)t->posvol, data, stride,
+                           rel->chunk_offset, rel->neighbor_size, rel->slice, (const KernelPoly<T> *)rel->poly,
+                           (T *)o->interp);
         LAUNCH_CHECK();
         return SPHX_OK;
     }
 };
-sphx_status interpolate_contact(sphx_relation *rel, int field) { return dispatch<InterpL>(rel->source, rel, field); }
+sphx_status interpolate_contact(sphx_relation *rel, int field, int component)
+{
+    return dispatch<InterpL>(rel->source, rel, field, component);
+}
 
 //---------------------------------------------------------------------------
 // kernel table -> per-interval cubic (host, long double)

@@ -313,6 +313,12 @@ sphx_status transfer(sphx_body *b, int field, void *host, bool write)
         return u32_dev((uint32_t *)b->prev_indicator);
     case SPHX_FIELD_POSITION_DIVERGENCE:
         return scalar_dev(b->pos_div);
+    case SPHX_FIELD_VISCOUS_FORCE:
+        return soa_dev(b->visc, T(0));
+    case SPHX_FIELD_PREVIOUS_VISCOUS_FORCE:
+        return soa_dev(b->prev_visc, T(0));
+    case SPHX_FIELD_KERNEL_GRADIENT_INTEGRAL:
+        return soa_dev(b->kgi, T(0));
     case SPHX_FIELD_ORIGINAL_ID:
         return u32_dev(b->original_id);
     case SPHX_FIELD_SORTED_ID:
@@ -496,6 +502,10 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
     if ((s = alloc(b.get(), (void **)&b->indicator, nf * 4)) ||
         (s = alloc(b.get(), (void **)&b->prev_indicator, nf * 4)) || (s = alloc(b.get(), &b->pos_div, nf * T)))
         return s;
+    for (int k = 0; k < b->D; ++k)
+        if ((s = alloc(b.get(), &b->visc[k], nf * T)) || (s = alloc(b.get(), &b->prev_visc[k], nf * T)) ||
+            (s = alloc(b.get(), &b->kgi[k], nf * T)))
+            return s;
     // zero every field, then the reference's initial values
     for (int k = 0; k < 3; ++k)
     {
@@ -542,7 +552,8 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
     {
         SPHX_HIP(hipMemsetAsync(b->pos_div, 0, nf * T, ctx->stream));
         for (int k = 0; k < b->D; ++k)
-            SPHX_HIP(hipMemsetAsync(b->grad_sum[k], 0, nf * T, ctx->stream));
+            for (void *f : {b->grad_sum[k], b->visc[k], b->prev_visc[k], b->kgi[k]})
+                SPHX_HIP(hipMemsetAsync(f, 0, nf * T, ctx->stream));
     }
     SPHX_HIP(hipStreamSynchronize(ctx->stream));
     *out = b.release();
@@ -576,10 +587,8 @@ static sphx_status body_rw(sphx_body *b, int field, void *host, size_t n, bool w
         return arg_error("particle count mismatch");
     if (field < 0 || field >= SPHX_FIELD_COUNT)
         return arg_error("unknown field id");
-    if ((field == SPHX_FIELD_CORRECTION_MATRIX || field == SPHX_FIELD_INDICATOR ||
-         field == SPHX_FIELD_PREVIOUS_INDICATOR || field == SPHX_FIELD_POSITION_DIVERGENCE) &&
-        b->desc.kind != SPHX_BODY_FLUID)
-        return arg_error("correction and surface-indication fields belong to fluid bodies");
+    if (field >= SPHX_FIELD_CORRECTION_MATRIX && b->desc.kind != SPHX_BODY_FLUID)
+        return arg_error("correction, surface-indication, viscous and transport fields belong to fluid bodies");
     if (b->n == 0)
         return SPHX_OK;
     SPHX_HIP(hipSetDevice(b->ctx->device));
@@ -975,6 +984,10 @@ sphx_status sphx_particle_sort(sphx_body *b, sphx_cell_linked_list *c)
     // evolving once a free-surface indication exists (surface_indication_ck.hpp:43)
     if (b->desc.kind == SPHX_BODY_FLUID && (s = gather_u32(b->ctx, (uint32_t *)b->prev_indicator, b->perm, b->n)))
         return s;
+    // evolving with a viscous force (force_prior_ck.cpp:14)
+    for (int k = 0; b->desc.kind == SPHX_BODY_FLUID && k < b->D; ++k)
+        if ((s = gather_real(b, &b->prev_visc[k], b->perm)))
+            return s;
     b->config_version++;
     return SPHX_OK;
 }
@@ -1091,6 +1104,42 @@ sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *co
     return free_surface_indication(inner, contact, n);
 }
 
+sphx_status sphx_viscous_force(sphx_relation *inner, sphx_relation *const *walls, int n, double mu, int correction)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n, true);
+    if (s)
+        return s;
+    if (!(mu >= 0.0) || !std::isfinite(mu))
+        return arg_error("viscosity must be finite and non-negative");
+    if (correction != SPHX_CORRECTION_NONE && correction != SPHX_CORRECTION_LINEAR)
+        return arg_error("unknown kernel correction");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return viscous_force(inner, walls, n, mu, correction == SPHX_CORRECTION_LINEAR);
+}
+
+sphx_status sphx_kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *contact, int n, int correction)
+{
+    sphx_status s = check_fluid_relations(inner, contact, n, true);
+    if (s)
+        return s;
+    if (correction != SPHX_CORRECTION_NONE && correction != SPHX_CORRECTION_LINEAR)
+        return arg_error("unknown kernel correction");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return kernel_gradient_integral(inner, contact, n, correction == SPHX_CORRECTION_LINEAR);
+}
+
+sphx_status sphx_transport_velocity_correction(sphx_body *b, double coefficient, int limiter, int bulk_only)
+{
+    if (!b || b->desc.kind != SPHX_BODY_FLUID)
+        return arg_error("transport-velocity correction needs a fluid body");
+    if (limiter != SPHX_LIMITER_NONE && limiter != SPHX_LIMITER_TRUNCATED_LINEAR)
+        return arg_error("unknown limiter");
+    if (!std::isfinite(coefficient))
+        return arg_error("coefficient must be finite");
+    SPHX_HIP(hipSetDevice(b->ctx->device));
+    return transport_velocity_correction(b, coefficient, limiter, bulk_only != 0);
+}
+
 // the max lanes: reset before a max reduction, combined when read
 static sphx_status reset_max(sphx_context *ctx)
 {
@@ -1178,11 +1227,27 @@ sphx_status sphx_interpolate_contact(sphx_relation *r, int field, double *out, s
         return state_error("observer relation not updated since the last configuration change");
     if (n_obs != r->source->n)
         return arg_error("observer count mismatch");
-    if (field != SPHX_FIELD_PRESSURE && field != SPHX_FIELD_DENSITY && field != SPHX_FIELD_COMPRESSION)
+    if (field != SPHX_FIELD_PRESSURE && field != SPHX_FIELD_DENSITY && field != SPHX_FIELD_COMPRESSION &&
+        field != SPHX_FIELD_VELOCITY)
         return arg_error("unsupported interpolation field");
     SPHX_HIP(hipSetDevice(r->source->ctx->device));
-    sphx_status s = interpolate_contact(r, field);
-    if (s)
+    sphx_status s;
+    if (field == SPHX_FIELD_VELOCITY)
+    {
+        // Interpolation<Contact<Vecd>>, one component at a time
+        const int D = r->source->D;
+        std::vector<double> comp(n_obs);
+        for (int k = 0; k < D; ++k)
+        {
+            if ((s = interpolate_contact(r, field, k)) ||
+                (s = sphx_body_read(r->source, SPHX_FIELD_INTERPOLATED, comp.data(), n_obs)))
+                return s;
+            for (size_t i = 0; i < n_obs; ++i)
+                out[i * D + k] = comp[i];
+        }
+        return SPHX_OK;
+    }
+    if ((s = interpolate_contact(r, field, 0)))
         return s;
     if ((s = sphx_body_read(r->source, SPHX_FIELD_INTERPOLATED, out, n_obs)))
         return s;

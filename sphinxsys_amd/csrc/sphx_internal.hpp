@@ -106,6 +106,10 @@ struct sphx_body
     // "Indicator", "PreviousSurfaceIndicator", "PositionDivergence"
     int32_t *indicator = nullptr, *prev_indicator = nullptr;
     void *pos_div = nullptr;
+    // "ViscousForce", "PreviousViscousForce" (evolving), "KernelGradientIntegral"
+    void *visc[3] = {nullptr, nullptr, nullptr};
+    void *prev_visc[3] = {nullptr, nullptr, nullptr};
+    void *kgi[3] = {nullptr, nullptr, nullptr};
     std::vector<void *> allocations;
 };
 
@@ -201,12 +205,16 @@ sphx_status acoustic_2nd_half_corrected(sphx_relation *inner, sphx_relation *con
                                         double dt, bool run_init, bool fuse_cfl);
 sphx_status linear_correction_matrix(sphx_relation *inner, sphx_relation *const *c, int nc, double alpha);
 sphx_status free_surface_indication(sphx_relation *inner, sphx_relation *const *c, int nc);
+sphx_status viscous_force(sphx_relation *inner, sphx_relation *const *w, int nw, double mu, bool corrected);
+sphx_status kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *c, int nc, bool corrected);
+sphx_status transport_velocity_correction(sphx_body *b, double coefficient, int limiter, bool bulk_only);
 sphx_status acoustic_signal_max(sphx_body *b, unsigned long long *slot);
 sphx_status advection_speed_max(sphx_body *b, unsigned long long *slot);
 sphx_status acoustic_dt_from_max(sphx_body *b, double cfl, const unsigned long long *slot, double *dt_out);
 sphx_status axis_max(sphx_body *b, int axis, unsigned long long *slot);
 sphx_status mechanical_energy(sphx_body *b, const double *g, double *out_device);
-sphx_status interpolate_contact(sphx_relation *rel, int field);
+// component: the velocity component for SPHX_FIELD_VELOCITY
+sphx_status interpolate_contact(sphx_relation *rel, int field, int component);
 sphx_status fill_u64(sphx_context *ctx, unsigned long long *p, size_t n, unsigned long long v);
 sphx_status build_kernel_poly(sphx_relation *rel);
 

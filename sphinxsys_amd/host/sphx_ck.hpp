@@ -87,7 +87,10 @@ class Error : public std::runtime_error
     X(sphx_linear_correction_matrix)                                                                      \
     X(sphx_acoustic_step_1st_half_corrected)                                                              \
     X(sphx_acoustic_step_2nd_half_corrected)                                                              \
-    X(sphx_free_surface_indication)
+    X(sphx_free_surface_indication)                                                                       \
+    X(sphx_viscous_force)                                                                                 \
+    X(sphx_kernel_gradient_integral)                                                                      \
+    X(sphx_transport_velocity_correction)
 
 class Api
 {
@@ -252,6 +255,9 @@ inline int field_id(const std::string &name)
         {"Indicator", SPHX_FIELD_INDICATOR},
         {"PreviousSurfaceIndicator", SPHX_FIELD_PREVIOUS_INDICATOR},
         {"PositionDivergence", SPHX_FIELD_POSITION_DIVERGENCE},
+        {"ViscousForce", SPHX_FIELD_VISCOUS_FORCE},
+        {"PreviousViscousForce", SPHX_FIELD_PREVIOUS_VISCOUS_FORCE},
+        {"KernelGradientIntegral", SPHX_FIELD_KERNEL_GRADIENT_INTEGRAL},
     };
     auto it = ids.find(name);
     if (it == ids.end())
@@ -263,7 +269,8 @@ inline bool is_vector_field(int id)
 {
     return id == SPHX_FIELD_POSITION || id == SPHX_FIELD_VELOCITY || id == SPHX_FIELD_FORCE ||
            id == SPHX_FIELD_FORCE_PRIOR || id == SPHX_FIELD_DISPLACEMENT || id == SPHX_FIELD_NORMAL ||
-           id == SPHX_FIELD_WALL_VELOCITY || id == SPHX_FIELD_WALL_ACCELERATION;
+           id == SPHX_FIELD_WALL_VELOCITY || id == SPHX_FIELD_WALL_ACCELERATION || id == SPHX_FIELD_VISCOUS_FORCE ||
+           id == SPHX_FIELD_PREVIOUS_VISCOUS_FORCE || id == SPHX_FIELD_KERNEL_GRADIENT_INTEGRAL;
 }
 inline bool is_index_field(int id)
 {
@@ -648,6 +655,59 @@ class LinearCorrectionMatrixComplex : public FluidInteraction
 
   private:
     double alpha_;
+};
+
+// ViscousForceCK<Inner<WithUpdate, Viscosity, K>, Contact<Wall, Viscosity, K>> (viscous_force.h:43-127)
+// with the body's constant viscosity; ViscousForceWithWallCK is K = NoKernelCorrectionCK
+template <int CORRECTION = SPHX_CORRECTION_NONE>
+class ViscousForceCK : public FluidInteraction
+{
+  public:
+    ViscousForceCK(Inner &inner, double mu) : FluidInteraction(inner, {}), mu_(mu) {}
+    ViscousForceCK(Inner &inner, Contact &wall, double mu) : FluidInteraction(inner, {&wall}), mu_(mu) {}
+    void interact()
+    {
+        api().check(api().sphx_viscous_force(inner_.handle(), walls(), (int)walls_.size(), mu_, CORRECTION));
+    }
+
+  private:
+    double mu_;
+};
+using ViscousForceWithWallCK = ViscousForceCK<SPHX_CORRECTION_NONE>;
+
+// KernelGradientIntegral<Inner<K>, Contact<Boundary, K>> (kernel_gradient_integral.h:89-93)
+template <int CORRECTION>
+class KernelGradientIntegralCK : public FluidInteraction
+{
+  public:
+    KernelGradientIntegralCK(Inner &inner) : FluidInteraction(inner, {}) {}
+    KernelGradientIntegralCK(Inner &inner, Contact &boundary) : FluidInteraction(inner, {&boundary}) {}
+    void interact()
+    {
+        api().check(api().sphx_kernel_gradient_integral(inner_.handle(), walls(), (int)walls_.size(), CORRECTION));
+    }
+};
+using KernelGradientIntegralComplex = KernelGradientIntegralCK<SPHX_CORRECTION_NONE>;
+using KernelGradientIntegralCorrectedComplex = KernelGradientIntegralCK<SPHX_CORRECTION_LINEAR>;
+
+// TransportVelocityCorrectionCK<SPHBody, Limiter[, BulkParticles]> (transport_velocity_correction_ck.h)
+template <int LIMITER = SPHX_LIMITER_NONE, bool BULK_ONLY = false>
+class TransportVelocityCorrectionCK
+{
+  public:
+    explicit TransportVelocityCorrectionCK(SPHBody &body, double coefficient = 0.2)
+        : body_(body), coefficient_(coefficient)
+    {
+    }
+    void update()
+    {
+        const Api &a = body_.api();
+        a.check(a.sphx_transport_velocity_correction(body_.handle(), coefficient_, LIMITER, BULK_ONLY ? 1 : 0));
+    }
+
+  private:
+    SPHBody &body_;
+    double coefficient_;
 };
 
 // FreeSurfaceIndicationCK<Inner<WithUpdate>, Contact<>> (surface_indication_ck.h:170)

@@ -46,6 +46,22 @@ def format_real(x, real_type=np.float64):
     return ("%.9g" if real_type == np.float32 else "%.17g") % x
 
 
+def format_value(v, real_type=np.float64):
+    """SimTK::String of a Real (format_real) or of a Vec: "~[" components "," "]"
+    in the stream's default format (%g, precision 6)."""
+    a = np.asarray(v, dtype=np.float64)
+    if a.ndim == 0:
+        return format_real(float(a), real_type)
+    return "~[" + ",".join("%g" % float(x) for x in a.reshape(-1)) + "]"
+
+
+def parse_value(text):
+    """A Real, or a Vec written as "~[a,b,...]" (returned as an array)."""
+    if text.startswith("~["):
+        return np.array([float(x) for x in text[2:-1].split(",")])
+    return float(text)
+
+
 def format_stream_default(x):
     """`std::ostream << double` with the default format (%g, precision 6)."""
     return "%g" % float(x)
@@ -61,11 +77,14 @@ def format_fixed9(x):
 # --------------------------------------------------------------------------
 class ObservedQuantityRecording:
     """Interpolated quantity at the observer points written to
-    `<observer>_<quantity>.dat` in `output_folder` on every write_to_file."""
+    `<observer>_<quantity>.dat` in `output_folder` on every write_to_file; a
+    vector quantity (`dim` components) writes "<quantity>[i][k]" columns
+    (io_plt.hpp:9-31)."""
 
-    def __init__(self, observe, n_observe, quantity_name, observer_name, output_folder):
+    def __init__(self, observe, n_observe, quantity_name, observer_name, output_folder, dim=None):
         self.observe = observe            # callable returning the observed values
         self.n = int(n_observe)
+        self.dim = dim
         self.quantity_name = quantity_name
         self.identifier = observer_name
         self.path = os.path.join(output_folder, f"{observer_name}_{quantity_name}.dat")
@@ -77,13 +96,19 @@ class ObservedQuantityRecording:
             with open(self.path, "w") as f:
                 f.write("run_time" + "   ")
                 for i in range(self.n):
-                    f.write('"' + f"{self.quantity_name}[{i}]" + '"' + "   ")
+                    name_i = f"{self.quantity_name}[{i}]"
+                    if self.dim is None:
+                        f.write('"' + name_i + '"' + "   ")
+                    else:
+                        for k in range(self.dim):
+                            f.write('"' + name_i + "[" + str(k) + ']"' + "   ")
                 f.write("\n")
             self.header_written = True
-        values = np.asarray(self.observe(), dtype=np.float64).reshape(self.n)
+        shape = (self.n,) if self.dim is None else (self.n, self.dim)
+        values = np.asarray(self.observe(), dtype=np.float64).reshape(shape)
         with open(self.path, "a") as f:
             f.write(format_stream_default(physical_time) + "   ")
-            for v in values:
+            for v in values.reshape(-1):
                 f.write(format_fixed9(v) + "   ")
             f.write("\n")
         self.last = values
@@ -132,7 +157,7 @@ def write_result_xml(path, series, real_type=np.float64):
            f'        <Snapshot number_of_snapshot_for_local_result_="{snapshots}" />\n',
            "    </Snapshot_Element>\n", "    <Result_Element>\n"]
     for k, s in enumerate(series):
-        attrs = " ".join(f'snapshot_{l}="{format_real(v, real_type)}"' for l, v in enumerate(s))
+        attrs = " ".join(f'snapshot_{l}="{format_value(v, real_type)}"' for l, v in enumerate(s))
         out.append(f"        <Particle_{k} {attrs} />\n")
     out += ["    </Result_Element>\n", "</result>\n"]
     with open(path, "w") as f:
@@ -145,7 +170,7 @@ def read_result_xml(path):
     snapshots = int(root.find("Snapshot_Element")[0].attrib["number_of_snapshot_for_local_result_"])
     series = []
     for el in root.find("Result_Element"):
-        series.append(np.array([float(el.attrib[f"snapshot_{l}"]) for l in range(snapshots)]))
+        series.append(np.array([parse_value(el.attrib[f"snapshot_{l}"]) for l in range(snapshots)]))
     return series
 
 
@@ -185,18 +210,24 @@ def dtw_distance(a, b, window=5):
     la, lb = len(a), len(b)
     if lb > 1.2 * la or lb < 0.8 * la:
         raise ValueError("series lengths differ by more than 20%")
+
+    def pnorm(x, y):
+        # calculatePNorm: |a - b| for a Real, (a - b).norm() for a vector (dynamic_time_warping_method.h:55-63)
+        diff = x - y
+        return np.abs(diff) if a.ndim == 1 else np.sqrt(np.sum(diff * diff, axis=-1))
+
     d = np.zeros((la, lb))
-    d[0, 0] = abs(a[0] - b[0])
+    d[0, 0] = pnorm(a[0], b[0])
     for i in range(1, la):
-        d[i, 0] = d[i - 1, 0] + abs(a[i] - b[0])
+        d[i, 0] = d[i - 1, 0] + pnorm(a[i], b[0])
     for j in range(1, lb):
-        d[0, j] = d[0, j - 1] + abs(a[0] - b[j])
+        d[0, j] = d[0, j - 1] + pnorm(a[0], b[j])
     window = max(window, abs(la - lb))
     for i in range(1, la):
         lo, hi = max(1, i - window), min(lb, i + window)
         if lo >= hi:
             continue
-        cost = np.abs(a[i] - b[lo:hi])
+        cost = pnorm(a[i], b[lo:hi])
         # the row recurrence has a left-neighbour dependence: walk it in order
         up = d[i - 1, lo:hi]
         diag = d[i - 1, lo - 1:hi - 1]
@@ -227,7 +258,8 @@ class RegressionTestDynamicTimeWarping:
             raise RuntimeError("the iteration step is not changed (duplicated observation)")
         self.last_iteration = iteration
         self.rec.write_to_file(physical_time, iteration)
-        self.current.append(np.array(self.rec.observed(), dtype=np.float64).reshape(self.rec.n))
+        obs = np.array(self.rec.observed(), dtype=np.float64)
+        self.current.append(obs.reshape((self.rec.n,) + obs.shape[1:]) if obs.ndim > 1 else obs.reshape(self.rec.n))
 
     def observed(self):
         return self.rec.observed()

@@ -2,7 +2,8 @@
 against the reference's archived files: every archived result, DTW and
 runtimes file written by the harness is reproduced byte for byte from its
 parsed values (double
-build: 2D and classic 3D dam break; float build: CK 3D dam break), the .dat
+build: 2D and classic 3D dam break; float build: CK 3D dam break and the
+lid-driven cavity's vector velocities), the .dat
 layouts against the stream formats of io_observation_ck.h / io_plt.cpp, the DTW
 recurrence against a direct transcription, and the test / database-generation
 procedures on the archived runs."""
@@ -20,6 +21,8 @@ ARCHIVES = [
     ("dambreak_3d_ck", "FluidObserver_Pressure", (0, 10, 20), np.float32),
     ("dambreak_3d_ck", "WaterBody_TotalMechanicalEnergy", (0, 5, 10), np.float32),
     ("dambreak_3d_classic", "FluidObserver_Pressure", (0, 9, 18), np.float64),
+    ("lid_driven_cavity", "HorizontalVelocity_Velocity", (0, 17, 33), np.float32),
+    ("lid_driven_cavity", "VerticalVelocity_Velocity", (0, 14, 28), np.float32),
 ]
 
 
