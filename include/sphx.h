@@ -246,7 +246,7 @@ sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *co
 sphx_status sphx_viscous_force(sphx_relation *inner, sphx_relation *const *walls, int n_wall, double mu,
                                int correction);
 /* reference: InteractionDynamicsCK<P, KernelGradientIntegral<Inner<K>, Contact<Boundary, K>>>
- * (general_dynamics/kernel_gradient_integral.hpp:39-100; KernelGradientIntegralComplex /
+ * (general_dynamics/kernel_gradient_integral.hpp:37-49,72-84; KernelGradientIntegralComplex /
  * ...CorrectedComplex): "KernelGradientIntegral" = -sum (B_i + B_j) dW V_j e_ij over the inner
  * neighbours - sum 2 B_i dW V_j e_ij over the contact neighbours. */
 sphx_status sphx_kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *contact,

@@ -612,7 +612,7 @@ static sphx_status viscous_force(sphx_relation *inner, sphx_relation *const *wal
     return SPHX_OK;
 }
 
-// KernelGradientIntegral<Inner<K>> + <Contact<Boundary, K>> (kernel_gradient_integral.hpp:39-100)
+// KernelGradientIntegral<Inner<K>> + <Contact<Boundary, K>> (kernel_gradient_integral.hpp:37-49,72-84)
 template <class R, int D>
 static sphx_status kernel_gradient_integral(sphx_relation *inner, sphx_relation *const *contact, int n_contact,
                                             bool corrected)

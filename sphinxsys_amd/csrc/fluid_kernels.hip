@@ -1007,7 +1007,7 @@ __global__ void __launch_bounds__(kThreads) k_viscous(FluidArgs<typename Prec<P>
 }
 
 //---------------------------------------------------------------------------
-// KernelGradientIntegral<Inner<K>> + <Contact<Boundary, K>> (kernel_gradient_integral.hpp:57-100):
+// KernelGradientIntegral<Inner<K>> + <Contact<Boundary, K>> (kernel_gradient_integral.hpp:37-49,72-84):
 // g_i = -sum_j (C_i + C_j) dW V_j e_ij - sum_w 2 C_i dW V_w e_iw
 //---------------------------------------------------------------------------
 template <int D, int P, bool CORR>
