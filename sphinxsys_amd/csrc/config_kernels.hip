@@ -389,53 +389,11 @@ __global__ void k_slice_to_csr(const uint32_t *count, const uint32_t *chunk, con
         csr[o + k] = slice[base + (k << 6)];
 }
 
-//---------------------------------------------------------------------------
-// LDS hash set of particle indices (linear probing), used for the staging sets
-//---------------------------------------------------------------------------
-constexpr uint32_t kHashSize = 4096;
 constexpr uint32_t kEmpty = 0xffffffffu;
-
-__device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 2654435761u) >> 20; }
-
-__device__ __forceinline__ uint32_t hash_find(const uint32_t *ht, uint32_t key)
-{
-    uint32_t s = hash_slot(key);
-    while (ht[s] != key)
-        s = (s + 1u) & (kHashSize - 1u);
-    return s;
-}
-
-// Collects the occupied slots of the block's hash set, in slot order, into
-// keys[0, total).  Returns false, with keys untouched, when the set overflowed
-// the hash or has more than `cap` entries.  Called by all threads of a
-// 256-thread block.
-__device__ bool block_collect(const uint32_t *ht, uint32_t *keys, uint32_t cap, uint32_t hash_full,
-                              uint32_t *total_out)
-{
-    const uint32_t t = threadIdx.x;
-    constexpr uint32_t per = kHashSize / kThreads;
-    uint32_t occ = 0;
-    for (uint32_t q = 0; q < per; ++q)
-        occ += ht[t * per + q] != kEmpty;
-    uint32_t total;
-    uint32_t pos = block_exclusive_scan(occ, &total);
-    *total_out = total;
-    if (total > cap || total > kStageMaxCap || hash_full)
-        return false;
-    for (uint32_t q = 0; q < per; ++q)
-    {
-        const uint32_t v = ht[t * per + q];
-        if (v != kEmpty)
-            keys[pos++] = v;
-    }
-    __syncthreads();
-    return true;
-}
 
 //---------------------------------------------------------------------------
 // one-pass relation build: a single candidate scan per source particle writes
-// the neighbour slices (fixed kmax rows per 64-particle chunk) and, for staged
-// relations, the block's distinct-neighbour set with the 16-bit ranks.
+// the neighbour slices (fixed kmax rows per 64-particle chunk).
 // Rows of a wave are assembled in an LDS ring buffer and written out whole, so
 // the list writes are coalesced rows instead of scattered 4-byte stores.
 //---------------------------------------------------------------------------
@@ -944,11 +902,13 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 // block of kStageBlock source particles the distinct neighbours (stage_keys,
 // stage_count) and every entry as a 16-bit rank into them (local_slice), or
 // kStageOverflow when the set exceeds the capacity.  A hash set in LDS
-// collects the distinct indices and a rank table maps each occupied slot to
-// its position in slot order.  Where keys collide the rank assignment depends
-// on the order of insertion; the passes use a rank only as an LDS address, so
-// their results do not.  (Sorting the set would make the staging loads of the
-// passes read nearby particles together, but costs more than it saves.)
+// collects the distinct indices in segments of 16 consecutive indices (one
+// 16-bit occupancy mask per segment), and an entry's rank is its segment's
+// base plus the set bits below it.  The passes use a rank only as an LDS
+// address, so their results do not depend on it; keeping runs of consecutive
+// indices in ascending order makes their staging loads read 256-byte runs and
+// lets neighbouring lanes read neighbouring LDS records (fewer bank conflicts)
+// without sorting the set.
 //---------------------------------------------------------------------------
 struct StageArgs
 {
@@ -958,20 +918,40 @@ struct StageArgs
     uint16_t *local_slice;
 };
 
+// segments of the staging set: 16 consecutive particle indices (a group) share
+// one segment, found by open addressing on the group id; the segment holds a
+// 16-bit occupancy mask, so a group's staged records keep ascending index order
+constexpr uint32_t kSegShift = 4;
+constexpr uint32_t kNumSeg = 1024;
+static_assert(kNumSeg % kThreads == 0, "segments are split evenly over the threads");
+
+__device__ __forceinline__ uint32_t seg_hash(uint32_t group) { return (group * 2654435761u) >> 22; }
+
+__device__ __forceinline__ uint32_t seg_find(const uint32_t *owner, uint32_t group)
+{
+    uint32_t s = seg_hash(group);
+    while (owner[s] != group)
+        s = (s + 1u) & (kNumSeg - 1u);
+    return s;
+}
+
 __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
 {
-    __shared__ uint32_t ht[kHashSize];
-    __shared__ uint32_t keys[kStageMaxCap];
-    __shared__ uint16_t rank[kHashSize];
-    __shared__ uint32_t hash_full;
+    __shared__ uint32_t owner[kNumSeg];
+    __shared__ uint32_t mask[kNumSeg];
+    __shared__ uint32_t seg_base[kNumSeg];
+    __shared__ uint32_t seg_full;
     const uint32_t t = threadIdx.x;
     const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + t;
     const bool valid = i < s.n_src;
-    for (uint32_t k = t; k < kHashSize; k += kThreads)
-        ht[k] = kEmpty;
+    for (uint32_t k = t; k < kNumSeg; k += kThreads)
+    {
+        owner[k] = kEmpty;
+        mask[k] = 0;
+    }
     if (t == 0)
-        hash_full = 0;
+        seg_full = 0;
     __syncthreads();
     const uint32_t count = valid ? s.neighbor_size[i] : 0u;
     const size_t row0 = valid ? (size_t)s.chunk_offset[i >> 6] + (i & 63u) : 0;
@@ -987,33 +967,51 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
         {
             if (jj[u] == kEmpty)
                 continue;
-            uint32_t h = hash_slot(jj[u]), probes = 0;
+            const uint32_t g = jj[u] >> kSegShift;
+            uint32_t h = seg_hash(g), probes = 0;
             while (true)
             {
-                const uint32_t old = atomicCAS(&ht[h], kEmpty, jj[u]);
-                if (old == kEmpty || old == jj[u])
-                    break;
-                h = (h + 1u) & (kHashSize - 1u);
-                if (++probes == kHashSize)
+                const uint32_t old = atomicCAS(&owner[h], kEmpty, g);
+                if (old == kEmpty || old == g)
                 {
-                    hash_full = 1;
+                    atomicOr(&mask[h], 1u << (jj[u] & ((1u << kSegShift) - 1u)));
+                    break;
+                }
+                h = (h + 1u) & (kNumSeg - 1u);
+                if (++probes == kNumSeg)
+                {
+                    seg_full = 1;
                     break;
                 }
             }
         }
     }
     __syncthreads();
-    uint32_t total = 0;
-    if (!block_collect(ht, keys, s.cap, hash_full, &total))
+    // compaction in segment order: rank base of each segment, then its keys
+    constexpr uint32_t per = kNumSeg / kThreads;
+    uint32_t occ = 0;
+    for (uint32_t q = 0; q < per; ++q)
+        occ += __popc(mask[t * per + q]);
+    uint32_t total;
+    uint32_t pos = block_exclusive_scan(occ, &total);
+    if (total > s.cap || seg_full) // block-uniform: both read after a barrier
     {
         if (t == 0)
             s.stage_count[blk] = kStageOverflow;
         return;
     }
-    for (uint32_t r = t; r < total; r += kThreads)
+    uint32_t *keys_out = s.stage_keys + (size_t)blk * s.cap;
+    for (uint32_t q = 0; q < per; ++q)
     {
-        rank[hash_find(ht, keys[r])] = (uint16_t)r;
-        s.stage_keys[(size_t)blk * s.cap + r] = keys[r];
+        const uint32_t sidx = t * per + q;
+        uint32_t m = mask[sidx];
+        seg_base[sidx] = pos;
+        const uint32_t first = owner[sidx] << kSegShift;
+        while (m)
+        {
+            keys_out[pos++] = first + (uint32_t)(__ffs(m) - 1);
+            m &= m - 1u;
+        }
     }
     __syncthreads();
     for (uint32_t r0 = 0; r0 < count; r0 += kInBatch)
@@ -1025,7 +1023,11 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
 #pragma unroll
         for (uint32_t u = 0; u < kInBatch; ++u)
             if (jj[u] != kEmpty)
-                s.local_slice[row0 + (size_t)(r0 + u) * 64u] = rank[hash_find(ht, jj[u])];
+            {
+                const uint32_t sidx = seg_find(owner, jj[u] >> kSegShift);
+                const uint32_t below = mask[sidx] & ((1u << (jj[u] & ((1u << kSegShift) - 1u))) - 1u);
+                s.local_slice[row0 + (size_t)(r0 + u) * 64u] = (uint16_t)(seg_base[sidx] + __popc(below));
+            }
     }
     if (t == 0)
         s.stage_count[blk] = total;

@@ -120,9 +120,9 @@ __device__ __forceinline__ typename Prec<P>::T full_C(typename Prec<P>::T stored
 
 // tabulated kernel polynomial in LDS
 template <class T>
-struct PolyLDS
+struct alignas(4 * sizeof(T)) PolyLDS
 {
-    T c[kPolyIntervals][4];
+    T c[kPolyIntervals][4]; // one interval's coefficients are one aligned 16/32-byte LDS read
 };
 
 template <class T>

@@ -33,7 +33,6 @@ struct Scratch
 constexpr int kScalarSlots = 1024;
 constexpr uint32_t kStageBlock = 256;          // source particles per staging block
 constexpr uint32_t kStageOverflow = 0xffffffffu;
-constexpr uint32_t kStageMaxCap = 2048;        // largest staged set the staging kernel collects
 constexpr uint32_t kStageCap = 1536;           // distinct neighbours staged per block (LDS records)
 // reported through the relation build's `needed` slot when a cell row holds
 // more candidates than the tiled build can code: rebuild with the scan kernel
