@@ -556,20 +556,18 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
 
 //---------------------------------------------------------------------------
 // tiled relation build: one wave per 64-particle chunk, four chunks per
-// workgroup.  The wave takes the chunk's source particles
-// one after another; its 64 lanes hold 64 consecutive candidates of each cell
-// row of the source's search box, and a ballot compacts the neighbours among
-// them, in candidate order, into the chunk's list tile in LDS (entry k of
-// chunk column c at tile[k][c]).  Sources in the same cell share their
-// candidates, which stay in registers until the cell changes.  The finished
-// tile is written out as whole 64-entry rows of the slice layout.  A tile
-// entry is 16 bits, (cell row << 12 | offset in the row), turned back into
-// the particle index when the rows are written.
+// workgroup.  The wave takes the chunk's source particles one after another;
+// its 64 lanes hold 64 consecutive candidates of the source's flat candidate
+// list (the cell rows of its search box, l -> m order, each row the
+// particles of up to three consecutive cells in cell order), and the
+// neighbours among them are recorded as one 64-bit lane mask per batch of 64
+// candidates, in LDS per source.  Sources in the same cell share their
+// candidates, which stay in registers until the cell changes.  The output
+// phase walks every lane's own masks in candidate order, one neighbour per
+// lane per slice row, so each slice row is written whole.
 //---------------------------------------------------------------------------
 constexpr uint32_t kTileWaves = 4;
-constexpr uint32_t kTileStride = 66; // uint16 per tile row: 64 + 2 so that rows start on different banks
-constexpr uint32_t kRowCodeBits = 12;
-constexpr uint16_t kNoSlot = 0xffffu;
+constexpr uint32_t kMaskSlots = 16;  // candidate batches of 64 per source: at most 1024 candidates
 constexpr float kFarAway = 1.0e30f; // position of an unused candidate slot (squared distance overflows to inf)
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t col)
@@ -588,12 +586,9 @@ __device__ __forceinline__ double lane_f64(double v, uint32_t col)
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), (int)col);
     return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
 }
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask, uint32_t add = 0u)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, add));
-}
 // lane mask of a per-lane condition (the compare result itself, no int round trip)
 __device__ __forceinline__ unsigned long long lane_mask(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+__device__ __forceinline__ bool lane_in(unsigned long long m, uint32_t lane) { return (m >> lane) & 1ull; }
 
 // v[idx] for a per-lane idx as a chain of selects kept in registers (the
 // empty asm stops the chain from being turned into a private-memory table)
@@ -616,7 +611,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
     constexpr int R = D == 3 ? 9 : 3; // cell rows of the search box, l -> m order
     constexpr int QMAX = 8;           // candidates of a cell kept in registers: 64 * QMAX
     const RelationArgs &a = b.r;
-    extern __shared__ uint16_t tiles[];
+    __shared__ unsigned long long masks[kTileWaves][64][kMaskSlots];
     __shared__ uint32_t any_dense;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     const uint32_t blk = xcd_block_index();
@@ -624,7 +619,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
     const uint32_t i = chunk * 64u + lane;
     const bool valid = i < a.n_src;
     const uint32_t kmax = b.kmax;
-    uint16_t *tile = tiles + (size_t)wave * kmax * kTileStride;
+    unsigned long long(&my_masks)[64][kMaskSlots] = masks[wave];
     if (t == 0)
         any_dense = 0;
     __syncthreads();
@@ -672,13 +667,12 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
             }
         }
     }
-    // an entry codes (cell row, offset in the row) in 16 bits: denser rows go
-    // to the scan kernel
-    bool dense = false;
+    uint32_t my_total = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        dense |= len[r] > (1u << kRowCodeBits);
-    if (__ballot(dense) && lane == 0)
+        my_total += len[r];
+    // a search box with more candidates than the masks hold goes to the scan kernel
+    if (__ballot(my_total > 64u * kMaskSlots) && lane == 0)
         any_dense = 1;
     __syncthreads();
     if (any_dense)
@@ -693,8 +687,8 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 #pragma unroll
     for (int r = 0; r < R; ++r)
         u_s0[r] = u_p[r] = 0;
-    // candidate f of the flat list: cell row, offset, particle index
-    auto locate = [&](uint32_t f, uint32_t &code, uint32_t &j) {
+    // particle index of candidate f of the flat list
+    auto locate = [&](uint32_t f) -> uint32_t {
         uint32_t row = 0, off = f;
 #pragma unroll
         for (int r = 1; r < R; ++r)
@@ -703,24 +697,19 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                 row = (uint32_t)r;
                 off = f - u_p[r];
             }
-        code = (row << kRowCodeBits) | off;
-        j = a.particle_index[pick<R>(u_s0, row) + off];
+        return a.particle_index[pick<R>(u_s0, row) + off];
     };
-    uint32_t cj[QMAX], ccode[QMAX];
+    uint32_t cj[QMAX];
     float cx[QMAX], cy[QMAX], cz[QMAX];
 #pragma unroll
     for (int q = 0; q < QMAX; ++q)
     {
-        cj[q] = ccode[q] = 0;
+        cj[q] = 0;
         cx[q] = cy[q] = cz[q] = 0.0f;
     }
     int cc0 = -1, cc1 = -1, cc2 = -1;
     uint32_t my_count = 0;
     // sources with candidates (most fluid particles have no wall cell near)
-    uint32_t my_total = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-        my_total += len[r];
     unsigned long long todo = __ballot(my_total > 0u);
     while (todo)
     {
@@ -745,7 +734,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
                 if (64u * q + lane < fill)
-                    locate(64u * q + lane, ccode[q], cj[q]);
+                    cj[q] = locate(64u * q + lane);
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
             {
@@ -782,50 +771,53 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
             }
             return r2 < 4.0;
         };
-        uint32_t k = 0; // neighbours of source col so far (wave-uniform)
+        // the pre-filter: below lo_thr a neighbour, below hi_thr the double
+        // criterion decides (exact sources: every listed candidate); the margin
+        // bounds the rounding of any evaluation order
+        const float lo_thr = s_exact ? -1.0f : a.thr_lo;
+        const float hi_thr = s_exact ? kFarAway : a.thr_hi;
+        uint32_t k = 0; // neighbours of source col (wave-uniform)
         if (u_total <= 64u * QMAX)
         {
-            // every batch of the cached candidates at once, free of branches
-            // (bitwise, not short-circuit, logic) so that the batches
-            // interleave: float squared distance, neighbour and margin-shell masks
-            bool hq[QMAX], sh[QMAX];
-            unsigned long long any_shell = 0;
+            // every batch of the cached candidates at once: neighbour and
+            // margin-shell lane masks
+            unsigned long long hit[QMAX], shell[QMAX], any_shell = 0;
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
             {
-                const bool other = cj[q] != self;
                 const float fx = sx - cx[q], fy = sy - cy[q], fz = D == 3 ? sz - cz[q] : 0.0f;
-                // the pre-filter margin bounds the rounding of any evaluation order
                 float r2f = fx * fx;
                 r2f = fmaf(fy, fy, r2f);
                 if (D == 3)
                     r2f = fmaf(fz, fz, r2f);
-                // exact lanes: every listed candidate goes to the shell test
-                const bool lo = r2f < a.thr_lo, hi = r2f < (s_exact ? kFarAway : a.thr_hi);
-                hq[q] = other & lo & !s_exact;
-                sh[q] = other & hi & !(lo & !s_exact);
-                any_shell |= lane_mask(sh[q]);
+                // one compare per mask, combined as scalar masks
+                const unsigned long long other = lane_mask(cj[q] != self);
+                const unsigned long long lo = lane_mask(r2f < lo_thr);
+                const unsigned long long hi = lane_mask(r2f < hi_thr);
+                hit[q] = other & lo;
+                shell[q] = other & hi & ~lo;
+                any_shell |= shell[q];
             }
             if (any_shell)
             {
                 source_position();
 #pragma unroll
                 for (int q = 0; q < QMAX; ++q)
-                    if (lane_mask(sh[q]))
+                    if (shell[q])
                     {
-                        if (sh[q])
-                            hq[q] = exact(cj[q]);
+                        bool in = false;
+                        if (lane_in(shell[q], lane))
+                            in = exact(cj[q]);
+                        hit[q] |= lane_mask(in);
                     }
             }
-            // compaction of the neighbours, in candidate order, into column col
+            const uint32_t nq = (u_total + 63u) >> 6;
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
             {
-                const unsigned long long m = lane_mask(hq[q]);
-                const uint32_t pos = lanes_below(m, k);
-                if (hq[q] & (pos < kmax))
-                    tile[pos * kTileStride + col] = (uint16_t)ccode[q];
-                k += (uint32_t)__popcll(m);
+                if ((uint32_t)q < nq && lane == 0)
+                    my_masks[col][q] = hit[q];
+                k += (uint32_t)__popcll(hit[q]);
             }
         }
         else
@@ -834,12 +826,10 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
             source_position();
             for (uint32_t f0 = 0; f0 < u_total; f0 += 64u)
             {
-                const bool act = f0 + lane < u_total;
-                uint32_t code = 0, j = kEmpty;
-                bool hit = false;
-                if (act)
+                bool in = false;
+                if (f0 + lane < u_total)
                 {
-                    locate(f0 + lane, code, j);
+                    const uint32_t j = locate(f0 + lane);
                     if (j != self)
                     {
                         const Rec4<float> p = a.tar_f[j];
@@ -848,16 +838,15 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                         r2f = fmaf(fy, fy, r2f);
                         if (D == 3)
                             r2f = fmaf(fz, fz, r2f);
-                        hit = r2f < a.thr_lo;
-                        if (s_exact || (r2f >= a.thr_lo && r2f < a.thr_hi))
-                            hit = exact(j);
+                        in = r2f < lo_thr;
+                        if (!in && r2f < hi_thr)
+                            in = exact(j);
                     }
                 }
-                const unsigned long long mask = __ballot(hit);
-                const uint32_t pos = k + lanes_below(mask);
-                if (hit && pos < kmax)
-                    tile[pos * kTileStride + col] = (uint16_t)code;
-                k += (uint32_t)__popcll(mask);
+                const unsigned long long m = lane_mask(in);
+                if (lane == 0)
+                    my_masks[col][f0 >> 6] = m;
+                k += (uint32_t)__popcll(m);
             }
         }
         if (lane == col)
@@ -870,10 +859,22 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
         if (my_count > kmax)
             atomicMax(b.needed, (unsigned long long)my_count);
     }
-    // rows out: decode this lane's entries (its own source's rows) and write
-    // the slice rows
+    // rows out: each lane walks its own masks in candidate order, one
+    // neighbour per slice row, so the rows are written whole
+    uint32_t P[R]; // this lane's flat-list offsets of its cell rows
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            P[r] = acc;
+            acc += len[r];
+        }
+    }
     const size_t row0 = (size_t)chunk * kmax * 64u + lane;
     const uint32_t rows = wave_max_u32(mine);
+    uint32_t cur_q = 0;
+    unsigned long long cur = mine > 0u ? my_masks[lane][0] : 0ull;
     constexpr uint32_t kOutBatch = 8; // index gathers in flight per lane
     for (uint32_t r0 = 0; r0 < rows; r0 += kOutBatch)
     {
@@ -881,12 +882,21 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
 #pragma unroll
         for (uint32_t u = 0; u < kOutBatch; ++u)
         {
-            const uint32_t r = r0 + u;
             jj[u] = kEmpty;
-            if (r < mine)
+            if (r0 + u < mine)
             {
-                const uint32_t e = tile[r * kTileStride + lane];
-                const uint32_t row = e >> kRowCodeBits, off = e & ((1u << kRowCodeBits) - 1u);
+                while (cur == 0ull) // the lane's next batch with a neighbour (mine bounds the walk)
+                    cur = my_masks[lane][++cur_q];
+                const uint32_t f = (cur_q << 6) + (uint32_t)(__ffsll((long long)cur) - 1);
+                cur &= cur - 1ull;
+                uint32_t row = 0, off = f;
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    if (f >= P[r])
+                    {
+                        row = (uint32_t)r;
+                        off = f - P[r];
+                    }
                 jj[u] = a.particle_index[pick<R>(s0, row) + off];
             }
         }
@@ -1242,21 +1252,18 @@ sphx_status relation_to_csr(sphx_relation *rel)
 }
 
 template <int D>
-static sphx_status launch_tile(sphx_context *ctx, const BuildArgs &b, uint32_t nchunks, size_t lds)
+static sphx_status launch_tile(sphx_context *ctx, const BuildArgs &b, uint32_t nchunks)
 {
-    const void *fn = reinterpret_cast<const void *>(&k_relation_tile<D>);
-    if (lds > 65536)
-        SPHX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_relation_tile<D>), dim3(grid_for(nchunks, kTileWaves)), dim3(kThreads), (unsigned)lds,
-                       ctx->stream, b);
+    hipLaunchKernelGGL((k_relation_tile<D>), dim3(grid_for(nchunks, kTileWaves)), dim3(kThreads), 0, ctx->stream,
+                       b);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
 // UpdateRelation in one pass: fixed kmax rows per chunk; *needed > kmax on
 // return asks the caller to grow kmax and run again, *needed == kTooDense to
-// run again with scan = true.  The tiled kernel runs when its LDS tile for
-// kmax rows fits the device, the per-lane scan kernel otherwise.
+// run again with scan = true (a search box with more candidates than the
+// tiled kernel's masks hold).
 sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed, bool scan)
 {
     uint32_t n = rel->source->n;
@@ -1270,11 +1277,10 @@ sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long
     b.r = relation_args(rel);
     b.kmax = kmax;
     b.needed = needed;
-    const size_t lds = (size_t)kTileWaves * kmax * kTileStride * sizeof(uint16_t);
-    if (!scan && lds <= ctx->max_lds)
+    if (!scan)
     {
         ProfileScope prof(ctx, "relation_build");
-        return rel->source->D == 2 ? launch_tile<2>(ctx, b, nchunks, lds) : launch_tile<3>(ctx, b, nchunks, lds);
+        return rel->source->D == 2 ? launch_tile<2>(ctx, b, nchunks) : launch_tile<3>(ctx, b, nchunks);
     }
     ProfileScope prof(ctx, "relation_scan");
     dim3 g(grid_for(n, kThreads)), blk(kThreads);

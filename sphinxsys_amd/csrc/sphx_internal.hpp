@@ -34,8 +34,8 @@ constexpr int kScalarSlots = 1024;
 constexpr uint32_t kStageBlock = 256;          // source particles per staging block
 constexpr uint32_t kStageOverflow = 0xffffffffu;
 constexpr uint32_t kStageCap = 1536;           // distinct neighbours staged per block (LDS records)
-// reported through the relation build's `needed` slot when a cell row holds
-// more candidates than the tiled build can code: rebuild with the scan kernel
+// reported through the relation build's `needed` slot when a search box holds
+// more candidates than the tiled build's masks: rebuild with the scan kernel
 constexpr unsigned long long kTooDense = 1ull << 40;
 enum ScalarSlot
 {
