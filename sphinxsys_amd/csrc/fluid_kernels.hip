@@ -17,6 +17,7 @@
 #include "sphx_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace sphx
 {
@@ -168,6 +169,99 @@ __device__ __forceinline__ void for_each_neighbor(const I *entries, uint32_t bas
         f((uint32_t)entries[base + (k << 6)]);
 }
 
+// Geometry of pair (i, j) handed to the pair functions of the pipelined walk:
+// r_ij = x_i - x_j, 1/|r_ij| and the tabulated kernel (value or gradient) at |r_ij|
+template <class T>
+struct PairGeo
+{
+    T dx, dy, dz, ir, w;
+};
+
+// one interval's cubic of the tabulated kernel, read as one vector from LDS
+template <class T>
+struct alignas(4 * sizeof(T)) Coef4
+{
+    T c0, c1, c2, c3;
+};
+
+// The neighbour walk of for_each_neighbor with the latencies overlapped: the
+// entries are taken in batches of 8; the list entries of the next batch are
+// loaded while the current one is evaluated, and for each batch the records
+// are fetched, the distances computed and the 8 kernel intervals requested
+// from LDS before any pair is finished.  fetch(entry) returns a record with
+// the neighbour's position and volume in .pv (entries past the count fetch
+// entry 0, which every non-empty list has, and are not finished); pair(record,
+// geo) finishes a pair.  Pairs are finished in list order with the arithmetic
+// of eval_poly, so the sums are those of the plain walk.
+template <int D, class T, class I, class Fetch, class Pair>
+__device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, uint32_t count,
+                                               const PV<T, D> &pi, const PolyLDS<T> &lds, T x_per_r,
+                                               Fetch &&fetch, Pair &&pair)
+{
+    constexpr uint32_t kBatch = 8;
+    const uint32_t nbatch = (count + kBatch - 1) / kBatch;
+    uint32_t next[kBatch];
+    auto load_entries = [&](uint32_t b) {
+        const I *row = entries + base + ((size_t)(b * kBatch) << 6);
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            next[q] = b * kBatch + q < count ? (uint32_t)row[q << 6] : 0u;
+    };
+    // one batch; the last one (FULL false) may hold fewer than kBatch entries
+    auto run_batch = [&](uint32_t b, auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        uint32_t idx[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            idx[q] = next[q];
+        if (b + 1 < nbatch)
+            load_entries(b + 1);
+        const uint32_t rem = count - b * kBatch;
+        decltype(fetch(0u)) rec[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            rec[q] = fetch(idx[q]);
+        PairGeo<T> g[kBatch];
+        int loc[kBatch];
+        T f[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+        {
+            const PV<T, D> &pj = rec[q].pv;
+            g[q].dx = pi.x - pj.x;
+            g[q].dy = pi.y - pj.y;
+            g[q].dz = D == 3 ? pi.z - pj.z : T(0);
+            const T r2 = D == 3 ? g[q].dx * g[q].dx + g[q].dy * g[q].dy + g[q].dz * g[q].dz
+                                : g[q].dx * g[q].dx + g[q].dy * g[q].dy;
+            g[q].ir = inv_sqrt(r2);
+            const T x = (r2 * g[q].ir) * x_per_r;
+            loc[q] = min((int)x, kPolyIntervals - 1);
+            f[q] = frac_part(x);
+        }
+        Coef4<T> c[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            c[q] = *reinterpret_cast<const Coef4<T> *>(lds.c[loc[q]]);
+        // keep the 8 reads issued together (the scheduler would otherwise sink
+        // each next to its use and wait on it alone)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            if (FULL || q < rem)
+            {
+                g[q].w = ((c[q].c3 * f[q] + c[q].c2) * f[q] + c[q].c1) * f[q] + c[q].c0;
+                pair(rec[q], g[q]);
+            }
+    };
+    const uint32_t nfull = count / kBatch;
+    if (nbatch > 0)
+        load_entries(0);
+    for (uint32_t b = 0; b < nfull; ++b)
+        run_batch(b, std::true_type{});
+    if (nfull < nbatch)
+        run_batch(nfull, std::false_type{});
+}
+
 //---------------------------------------------------------------------------
 // AcousticStep1stHalf::InitializeKernel (acoustic_step_1st_half.hpp:66-74)
 //---------------------------------------------------------------------------
@@ -283,24 +377,28 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     const T p_i = a.p[i];
     // inner: force_sum -= 2 AverageP dW V_j e_ij, AverageP = (p_i + p_j)/2 for one fluid
     T fx = 0, fy = 0, fz = 0, cd = 0;
-    auto pair = [&](const PV<T, D> &pj, const T p_j) {
-        const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
-        const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
-        const T ir = inv_sqrt(r2);
-        const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
-        const T s = (CORR ? p_j : p_i + p_j) * dWV * ir;
-        fx += s * dx;
-        fy += s * dy;
+    struct Rec
+    {
+        PV<T, D> pv;
+        T p;
+    };
+    auto pair = [&](const Rec &rj, const PairGeo<T> &g) {
+        const T dWV = g.w * rj.pv.vol;
+        const T s = (CORR ? rj.p : p_i + rj.p) * dWV * g.ir;
+        fx += s * g.dx;
+        fy += s * g.dy;
         if (D == 3)
-            fz += s * dz;
+            fz += s * g.dz;
         if (RIEMANN)
-            cd += (p_i - p_j) * dWV;
+            cd += (p_i - rj.p) * dWV;
     };
     const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
     if (staged)
-        for_each_neighbor(a.local_slice, nb, nc, [&](uint32_t r) { pair(load_pv<T, D>(s_pv, r), s_p[r]); });
+        walk_pipelined<D>(a.local_slice, nb, nc, pi, poly_in, x_per_r,
+                          [&](uint32_t r) { return Rec{load_pv<T, D>(s_pv, r), s_p[r]}; }, pair);
     else
-        for_each_neighbor(a.slice, nb, nc, [&](uint32_t j) { pair(load_pv<T, D>(a.posvol, j), a.p[j]); });
+        walk_pipelined<D>(a.slice, nb, nc, pi, poly_in, x_per_r,
+                          [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.p[j]}; }, pair);
     const T k_avg = T(2) * a.inv_rho0c0_sum * a.rho0c0; // 2 * AverageP / (p_i + p_j)
     const T vol_i = pi.vol;
     T Fx = a.force[0][i], Fy = a.force[1][i], Fz = D == 3 ? a.force[2][i] : T(0);
@@ -449,12 +547,16 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
             Bi.load(a.corr, i);
         // divergence: 2 (v_i - v_ave).e dW V = (v_i - v_j).e dW V for one fluid
         T div = 0, px = 0, py = 0, pz = 0;
-        auto pair = [&](const PV<T, D> &pj, const Rec4<T> &vj) {
-            const T dx = pi.x - pj.x, dy = pi.y - pj.y, dz = D == 3 ? pi.z - pj.z : T(0);
-            const T r2 = D == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
-            const T ir = inv_sqrt(r2);
-            const T dWV = eval_poly(poly_in, r2 * ir, x_per_r) * pj.vol;
-            const T ex = dx * ir, ey = dy * ir, ez = D == 3 ? dz * ir : T(0);
+        struct Rec
+        {
+            PV<T, D> pv;
+            Rec4<T> v;
+        };
+        auto pair = [&](const Rec &rj, const PairGeo<T> &g) {
+            const Rec4<T> &vj = rj.v;
+            const T dWV = g.w * rj.pv.vol;
+            const T ir = g.ir;
+            const T ex = g.dx * ir, ey = g.dy * ir, ez = D == 3 ? g.dz * ir : T(0);
             T u = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey;
             if (D == 3)
                 u += (vi.z - vj.z) * ez;
@@ -480,9 +582,11 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         };
         const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
         if (staged)
-            for_each_neighbor(a.local_slice, nb, nc, [&](uint32_t r) { pair(load_pv<T, D>(s_pv, r), s_v[r]); });
+            walk_pipelined<D>(a.local_slice, nb, nc, pi, poly_in, x_per_r,
+                              [&](uint32_t r) { return Rec{load_pv<T, D>(s_pv, r), s_v[r]}; }, pair);
         else
-            for_each_neighbor(a.slice, nb, nc, [&](uint32_t j) { pair(load_pv<T, D>(a.posvol, j), a.vel[j]); });
+            walk_pipelined<D>(a.slice, nb, nc, pi, poly_in, x_per_r,
+                              [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.vel[j]}; }, pair);
         const T C_i = full_C<P>(a.C[i]);
         const T vol_i = pi.vol;
         T dC = a.dC[i] + div * C_i;
