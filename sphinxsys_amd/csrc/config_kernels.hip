@@ -259,19 +259,83 @@ __device__ __forceinline__ void load_pos(const PosPtrs &p, uint32_t i, double *x
 
 template <int D>
 __global__ void __launch_bounds__(kThreads) k_cell_keys(PosPtrs pos, uint32_t n, MeshParams m, uint32_t *keys,
-                                                        uint32_t *vals, uint32_t *counts)
+                                                        uint32_t *counts)
 {
-    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n)
-        return;
-    double x[3];
-    load_pos<D>(pos, i, x);
-    int c[3];
-    cell_index<D>(m, x, c);
-    uint32_t lin = linear_cell<D>(m, c);
-    keys[i] = lin;
-    vals[i] = i;
-    atomicAdd(&counts[lin], 1u);
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x, lane = threadIdx.x & 63u;
+    uint32_t lin = 0xffffffffu;
+    if (i < n)
+    {
+        double x[3];
+        load_pos<D>(pos, i, x);
+        int c[3];
+        cell_index<D>(m, x, c);
+        lin = linear_cell<D>(m, c);
+        keys[i] = lin;
+    }
+    // one atomic per run of equal cells in the wave (particles are in cell
+    // order but for those that moved since the last sort)
+    const uint32_t prev = (uint32_t)__shfl_up((int)lin, 1u, 64);
+    const bool head = lane == 0u || prev != lin;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long above = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
+    const uint32_t next = above ? (uint32_t)__ffsll((long long)above) - 1u : 64u;
+    if (head && i < n)
+        atomicAdd(&counts[lin], next - lane);
+}
+
+// runs of equal cell keys in particle order: flag[i] = 1 where a run starts
+__global__ void k_run_flags(const uint32_t *keys, uint32_t n, uint32_t *flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flag[i] = (i == 0u || keys[i] != keys[i - 1u]) ? 1u : 0u;
+}
+
+// run r (ex[i] at its first particle i): its key, its number and its start
+__global__ void k_run_emit(const uint32_t *keys, const uint32_t *ex, uint32_t n, uint32_t *run_key,
+                           uint32_t *run_idx, uint32_t *run_start)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (i == 0u || keys[i] != keys[i - 1u]))
+    {
+        const uint32_t r = ex[i];
+        run_key[r] = keys[i];
+        run_idx[r] = r;
+        run_start[r] = i;
+    }
+}
+
+// length of the run at sorted position s
+__global__ void k_run_len(const uint32_t *run_idx, const uint32_t *run_start, uint32_t nruns, uint32_t n,
+                          uint32_t *len)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nruns)
+    {
+        const uint32_t r = run_idx[s];
+        len[s] = (r + 1u < nruns ? run_start[r + 1u] : n) - run_start[r];
+    }
+}
+
+// first output slot of each run, by run number
+__global__ void k_run_off(const uint32_t *run_idx, const uint32_t *soff, uint32_t nruns, uint32_t *run_off)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nruns)
+        run_off[run_idx[s]] = soff[s];
+}
+
+// particle i goes to its run's first slot plus its place in the run
+__global__ void k_run_place(const uint32_t *keys, const uint32_t *ex, const uint32_t *run_start,
+                            const uint32_t *run_off, uint32_t n, uint32_t *particle_index)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        const uint32_t head = (i == 0u || keys[i] != keys[i - 1u]) ? 1u : 0u;
+        const uint32_t r = ex[i] + head - 1u;
+        particle_index[run_off[r] + (i - run_start[r])] = i;
+    }
 }
 
 template <int D>
@@ -1165,22 +1229,47 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
     const size_t nc = cll->ncells;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nc + 1, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,
                        nc + 1, 0u);
-    if (b->n > 0)
+    const uint32_t n = b->n;
+    if (n > 0)
     {
         // counts accumulate in cell_offset, then scan in place
         if (b->D == 2)
-            hipLaunchKernelGGL(k_cell_keys<2>, dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                               pos_ptrs(b), b->n, cll->mesh, cll->keys, cll->particle_index, cll->cell_offset);
+            hipLaunchKernelGGL(k_cell_keys<2>, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
+                               pos_ptrs(b), n, cll->mesh, cll->keys, cll->cell_offset);
         else
-            hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                               pos_ptrs(b), b->n, cll->mesh, cll->keys, cll->particle_index, cll->cell_offset);
+            hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
+                               pos_ptrs(b), n, cll->mesh, cll->keys, cll->cell_offset);
         SPHX_HIP(hipGetLastError());
     }
     sphx_status s = exclusive_scan_u32(ctx, cll->cell_offset, cll->cell_offset, nc + 1, nullptr);
-    if (s)
+    if (s || n == 0)
         return s;
-    if ((s = radix_sort_pairs(ctx, cll->keys, cll->particle_index, b->n, cll->key_bits)))
+    // particles by cell, ascending index within a cell: the runs of equal
+    // cells in particle order (as many as cells, plus two per particle that
+    // left its run since the last sort) are sorted stably by cell, and each
+    // particle is placed at its run's output offset plus its place in the run
+    if ((s = ctx->cll_runs.reserve((size_t)6 * n * sizeof(uint32_t))))
         return s;
+    uint32_t *ex = (uint32_t *)ctx->cll_runs.ptr;
+    uint32_t *run_key = ex + n, *run_idx = run_key + n, *run_start = run_idx + n, *len = run_start + n,
+             *run_off = len + n;
+    const dim3 g(grid_for(n, 256)), blk(256);
+    hipLaunchKernelGGL(k_run_flags, g, blk, 0, ctx->stream, cll->keys, n, ex);
+    uint32_t nruns = 0;
+    if ((s = exclusive_scan_u32(ctx, ex, ex, n, &nruns)))
+        return s;
+    hipLaunchKernelGGL(k_run_emit, g, blk, 0, ctx->stream, cll->keys, ex, n, run_key, run_idx, run_start);
+    SPHX_HIP(hipGetLastError());
+    if ((s = radix_sort_pairs(ctx, run_key, run_idx, nruns, cll->key_bits)))
+        return s;
+    const dim3 gr(grid_for(nruns, 256));
+    hipLaunchKernelGGL(k_run_len, gr, blk, 0, ctx->stream, run_idx, run_start, nruns, n, len);
+    if ((s = exclusive_scan_u32(ctx, len, len, nruns, nullptr)))
+        return s;
+    hipLaunchKernelGGL(k_run_off, gr, blk, 0, ctx->stream, run_idx, len, nruns, run_off);
+    hipLaunchKernelGGL(k_run_place, g, blk, 0, ctx->stream, cll->keys, ex, run_start, run_off, n,
+                       cll->particle_index);
+    SPHX_HIP(hipGetLastError());
     cll->built = true;
     ++cll->build_count;
     return SPHX_OK;

@@ -63,7 +63,7 @@ struct sphx_context
     std::map<std::string, std::pair<double, uint64_t>> profile; // stage -> (ms, launches)
     unsigned long long *scalars_d = nullptr; // kScalarSlots u64 on the device
     unsigned long long *scalars_h = nullptr; // pinned mirror
-    sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp;
+    sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp, cll_runs;
 };
 
 struct sphx_body
