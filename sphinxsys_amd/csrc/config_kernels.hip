@@ -771,28 +771,28 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
         cj[q] = 0;
         cx[q] = cy[q] = cz[q] = 0.0f;
     }
-    int cc0 = -1, cc1 = -1, cc2 = -1;
     uint32_t my_count = 0;
     // sources with candidates (most fluid particles have no wall cell near)
+    // sources are taken cell by cell (all of a cell's sources in the chunk
+    // share its candidates), so a particle that left its cell's run since the
+    // last sort does not make the wave reload the candidates
     unsigned long long todo = __ballot(my_total > 0u);
+    unsigned long long group = 0ull; // sources of the current cell not yet done
     while (todo)
     {
-        const uint32_t col = (uint32_t)__ffsll((long long)todo) - 1u;
-        todo &= todo - 1ull;
-        const int q0 = lane_i32(c[0], col), q1 = lane_i32(c[1], col), q2 = D == 3 ? lane_i32(c[2], col) : 0;
-        if (q0 != cc0 || q1 != cc1 || q2 != cc2)
+        if (!group)
         {
-            // a new cell: its candidates into registers
-            cc0 = q0;
-            cc1 = q1;
-            cc2 = q2;
+            // the next cell: its sources and its candidates into registers
+            const uint32_t lead = (uint32_t)__ffsll((long long)todo) - 1u;
+            const int q0 = lane_i32(c[0], lead), q1 = lane_i32(c[1], lead), q2 = D == 3 ? lane_i32(c[2], lead) : 0;
+            group = __ballot(c[0] == q0 && c[1] == q1 && (D == 2 || c[2] == q2)) & todo;
             u_total = 0;
 #pragma unroll
             for (int r = 0; r < R; ++r)
             {
-                u_s0[r] = lane_u32(s0[r], col);
+                u_s0[r] = lane_u32(s0[r], lead);
                 u_p[r] = u_total;
-                u_total += lane_u32(len[r], col);
+                u_total += lane_u32(len[r], lead);
             }
             const uint32_t fill = u_total <= 64u * QMAX ? u_total : 0u;
 #pragma unroll
@@ -811,6 +811,9 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                 cz[q] = p.z;
             }
         }
+        const uint32_t col = (uint32_t)__ffsll((long long)group) - 1u;
+        group &= group - 1ull;
+        todo &= ~(1ull << col);
         const uint32_t si = first_i + col;
         const uint32_t self = a.inner ? si : kEmpty;
         const float sx = lane_f32(xf.x, col), sy = lane_f32(xf.y, col), sz = D == 3 ? lane_f32(xf.z, col) : 0.0f;
