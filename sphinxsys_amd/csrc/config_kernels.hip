@@ -1033,6 +1033,31 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
     const uint32_t count = valid ? s.neighbor_size[i] : 0u;
     const size_t row0 = valid ? (size_t)s.chunk_offset[i >> 6] + (i & 63u) : 0;
     constexpr uint32_t kInBatch = 8; // row reads in flight per lane
+    constexpr uint32_t kLow = (1u << kSegShift) - 1u;
+    // a list runs through its candidates in cell order, ascending index within
+    // a cell: consecutive entries of one group are merged into one mask and
+    // inserted once
+    uint32_t g_cur = kEmpty, m_cur = 0;
+    auto insert = [&]() {
+        uint32_t h = seg_hash(g_cur), probes = 0;
+        while (true)
+        {
+            uint32_t old = owner[h];
+            if (old == kEmpty)
+                old = atomicCAS(&owner[h], kEmpty, g_cur);
+            if (old == kEmpty || old == g_cur)
+            {
+                atomicOr(&mask[h], m_cur);
+                break;
+            }
+            h = (h + 1u) & (kNumSeg - 1u);
+            if (++probes == kNumSeg)
+            {
+                seg_full = 1;
+                break;
+            }
+        }
+    };
     for (uint32_t r0 = 0; r0 < count; r0 += kInBatch)
     {
         uint32_t jj[kInBatch];
@@ -1045,24 +1070,18 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
             if (jj[u] == kEmpty)
                 continue;
             const uint32_t g = jj[u] >> kSegShift;
-            uint32_t h = seg_hash(g), probes = 0;
-            while (true)
+            if (g != g_cur)
             {
-                const uint32_t old = atomicCAS(&owner[h], kEmpty, g);
-                if (old == kEmpty || old == g)
-                {
-                    atomicOr(&mask[h], 1u << (jj[u] & ((1u << kSegShift) - 1u)));
-                    break;
-                }
-                h = (h + 1u) & (kNumSeg - 1u);
-                if (++probes == kNumSeg)
-                {
-                    seg_full = 1;
-                    break;
-                }
+                if (g_cur != kEmpty)
+                    insert();
+                g_cur = g;
+                m_cur = 0;
             }
+            m_cur |= 1u << (jj[u] & kLow);
         }
     }
+    if (g_cur != kEmpty)
+        insert();
     __syncthreads();
     // compaction in segment order: rank base of each segment, then its keys
     constexpr uint32_t per = kNumSeg / kThreads;
@@ -1091,6 +1110,8 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
         }
     }
     __syncthreads();
+    g_cur = kEmpty;
+    uint32_t base_cur = 0;
     for (uint32_t r0 = 0; r0 < count; r0 += kInBatch)
     {
         uint32_t jj[kInBatch];
@@ -1101,9 +1122,16 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
         for (uint32_t u = 0; u < kInBatch; ++u)
             if (jj[u] != kEmpty)
             {
-                const uint32_t sidx = seg_find(owner, jj[u] >> kSegShift);
-                const uint32_t below = mask[sidx] & ((1u << (jj[u] & ((1u << kSegShift) - 1u))) - 1u);
-                s.local_slice[row0 + (size_t)(r0 + u) * 64u] = (uint16_t)(seg_base[sidx] + __popc(below));
+                const uint32_t g = jj[u] >> kSegShift;
+                if (g != g_cur)
+                {
+                    const uint32_t sidx = seg_find(owner, g);
+                    base_cur = seg_base[sidx];
+                    m_cur = mask[sidx];
+                    g_cur = g;
+                }
+                const uint32_t below = m_cur & ((1u << (jj[u] & kLow)) - 1u);
+                s.local_slice[row0 + (size_t)(r0 + u) * 64u] = (uint16_t)(base_cur + __popc(below));
             }
     }
     if (t == 0)
