@@ -631,7 +631,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
 // lane per slice row, so each slice row is written whole.
 //---------------------------------------------------------------------------
 constexpr uint32_t kTileWaves = 4;
-constexpr uint32_t kMaskSlots = 16;  // candidate batches of 64 per source: at most 1024 candidates
+constexpr uint32_t kMaskSlots = 15;  // candidate batches of 64 per source: at most 960 candidates (30 KB of masks per block)
 constexpr float kFarAway = 1.0e30f; // position of an unused candidate slot (squared distance overflows to inf)
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t col)
@@ -795,10 +795,21 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                 u_total += lane_u32(len[r], lead);
             }
             const uint32_t fill = u_total <= 64u * QMAX ? u_total : 0u;
+            // slot f = 64 q + lane of the flat list: row by row, only the
+            // (row, batch) pairs that overlap (wave-uniform tests) do work
 #pragma unroll
-            for (int q = 0; q < QMAX; ++q)
-                if (64u * q + lane < fill)
-                    cj[q] = locate(64u * q + lane);
+            for (int r = 0; r < R; ++r)
+            {
+                const uint32_t a0 = u_p[r], a1 = r + 1 < R ? u_p[r + 1] : fill;
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q)
+                    if (a0 < a1 && 64u * q < a1 && 64u * q + 64u > a0)
+                    {
+                        const uint32_t f = 64u * q + lane;
+                        if (f >= a0 && f < a1)
+                            cj[q] = a.particle_index[u_s0[r] + (f - a0)];
+                    }
+            }
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
             {
@@ -864,6 +875,7 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                 hit[q] = other & lo;
                 shell[q] = other & hi & ~lo;
                 any_shell |= shell[q];
+                k += (uint32_t)__popcll(hit[q]);
             }
             if (any_shell)
             {
@@ -875,17 +887,16 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                         bool in = false;
                         if (lane_in(shell[q], lane))
                             in = exact(cj[q]);
-                        hit[q] |= lane_mask(in);
+                        const unsigned long long add = lane_mask(in); // shell bits are not in hit
+                        hit[q] |= add;
+                        k += (uint32_t)__popcll(add);
                     }
             }
             const uint32_t nq = (u_total + 63u) >> 6;
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
-            {
                 if ((uint32_t)q < nq && lane == 0)
                     my_masks[col][q] = hit[q];
-                k += (uint32_t)__popcll(hit[q]);
-            }
         }
         else
         {
