@@ -198,7 +198,7 @@ __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, 
                                                const PV<T, D> &pi, const PolyLDS<T> &lds, T x_per_r,
                                                Fetch &&fetch, Pair &&pair)
 {
-    constexpr uint32_t kBatch = 8;
+    constexpr uint32_t kBatch = 8; // 4 (double: 4 waves per SIMD instead of 2) measured slower
     const uint32_t nbatch = (count + kBatch - 1) / kBatch;
     uint32_t next[kBatch];
     auto load_entries = [&](uint32_t b) {
@@ -505,9 +505,19 @@ __device__ __forceinline__ T p_jump_scaled(const FluidArgs<T> &a, T u)
 // AcousticTimeStepCK for the next sub-step, max-reduced into the max lanes at max_slot.
 //---------------------------------------------------------------------------
 // CORR (LinearCorrectionCK): the divergence uses B_i e_ij in place of e_ij
+// Pass 2 stages at most kPass2Cap records, as (x, y, z|Vol, Vol|0) plus the
+// three velocity components in separate arrays (28 B a record), so that with
+// the static tables a block takes at most 40 KB of LDS and four blocks fit a
+// CU; a block with a larger staged set reads its neighbours from global
+// memory in this pass only.
+constexpr uint32_t kPass2Cap = 1400;
+static_assert(kPass2Cap <= kStageCap, "pass 2 stages a prefix of the staging capacity");
+
+// FP32-mixed: registers capped for 4 waves per SIMD, which its 40 KB of LDS
+// per block allows; FP64 (not staged) keeps its registers
 template <int D, int P, bool RIEMANN, bool FUSE_CFL, bool CORR>
-__global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d,
-                                                    unsigned long long *max_slot)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 4 : 1)))
+k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_slot)
 {
     using T = typename Prec<P>::T;
     __shared__ PolyLDS<T> poly_in;
@@ -519,11 +529,11 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         stage_poly(poly_w[k], a.wall[k].poly, true);
     const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + threadIdx.x;
-    // staged neighbour records: (x, y, z|Vol, Vol|0) and velocity
+    // staged neighbour records: (x, y, z|Vol, Vol|0) and the velocity components
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
-    Rec4<T> *s_v = s_pv + kStageCap;
+    T *s_vx = (T *)(s_pv + kPass2Cap), *s_vy = s_vx + kPass2Cap, *s_vz = s_vy + kPass2Cap;
     const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
-    const bool staged = n_staged != kStageOverflow;
+    const bool staged = n_staged <= kPass2Cap; // kStageOverflow is above any cap
     if (staged)
     {
         const uint32_t *keys = a.stage_keys + (size_t)blk * a.stage_cap;
@@ -531,7 +541,11 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         {
             const uint32_t j = keys[r];
             s_pv[r] = a.posvol[j];
-            s_v[r] = a.vel[j];
+            const Rec4<T> v = a.vel[j];
+            s_vx[r] = v.x;
+            s_vy[r] = v.y;
+            if (D == 3)
+                s_vz[r] = v.z;
         }
     }
     __syncthreads();
@@ -583,7 +597,11 @@ __global__ void __launch_bounds__(kThreads) k_pass2(FluidArgs<typename Prec<P>::
         const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
         if (staged)
             walk_pipelined<D>(a.local_slice, nb, nc, pi, poly_in, x_per_r,
-                              [&](uint32_t r) { return Rec{load_pv<T, D>(s_pv, r), s_v[r]}; }, pair);
+                              [&](uint32_t r) {
+                                  return Rec{load_pv<T, D>(s_pv, r),
+                                             Rec4<T>{s_vx[r], s_vy[r], D == 3 ? s_vz[r] : T(0), T(0)}};
+                              },
+                              pair);
         else
             walk_pipelined<D>(a.slice, nb, nc, pi, poly_in, x_per_r,
                               [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.vel[j]}; }, pair);
@@ -1843,7 +1861,7 @@ struct Acoustic2L
         ProfileScope prof(b->ctx, "acoustic_pass2");
         const bool rs = riemann != SPHX_RIEMANN_NONE;
         const bool cfl = max_slot != nullptr;
-        const size_t lds = a.stage_count ? (size_t)a.stage_cap * 2 * sizeof(Rec4<T>) : 0;
+        const size_t lds = a.stage_count ? (size_t)kPass2Cap * (sizeof(Rec4<T>) + 3 * sizeof(T)) : 0;
         SPHX_PASS_VARIANTS(k_pass2, rs, cfl, corr, a, dt, max_slot);
         LAUNCH_CHECK();
         return SPHX_OK;
