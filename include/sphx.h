@@ -319,7 +319,10 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
  * of the FP32-mixed inner relation), "cell_linked_list", "particle_sort",
  * "time_step_reduce", "state_update", "linear_correction" (correction matrix and
  * its neighbour sum), "surface_indication".  Reading synchronises the stream and
- * returns the accumulated milliseconds and launch count since enabling. */
+ * returns the accumulated milliseconds and launch count since enabling.
+ * enable: 0 off, 1 every stage, 2 only "acoustic_pass1" and "acoustic_pass2"
+ * (each event pair delays the next launch by a few microseconds, so timing
+ * every small kernel of a step slows the step by 2-3 %). */
 sphx_status sphx_profile_enable(sphx_context *ctx, int enable);
 sphx_status sphx_profile_read(sphx_context *ctx, const char *stage, double *total_ms,
                               uint64_t *launches);

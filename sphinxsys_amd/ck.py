@@ -37,9 +37,10 @@ class Context:
     def synchronize(self):
         self.lib.call("sphx_context_synchronize", self.h)
 
-    def profile(self, enable=True):
-        """Start (or stop) HIP-event timing of the library's kernel stages."""
-        self.lib.call("sphx_profile_enable", self.h, 1 if enable else 0)
+    def profile(self, enable=True, passes_only=False):
+        """Start (or stop) HIP-event timing of the library's kernel stages (all of
+        them, or only the two acoustic passes)."""
+        self.lib.call("sphx_profile_enable", self.h, (2 if passes_only else 1) if enable else 0)
 
     def profile_read(self, stage):
         """(total milliseconds, launches) of one stage since profile() was called."""

@@ -94,7 +94,23 @@ struct FluidArgs
     T *pos_div;
     // viscous force, its previous value in ForcePrior, kernel gradient integral
     T *visc[3], *prev_visc[3], *kgi[3];
+    // device-side sub-step control (null: dt is the kernel argument): the
+    // acoustic kernels return at once when STEP_ACTIVE is 0 and take STEP_DT
+    const unsigned long long *step;
 };
+
+// dt of an acoustic kernel: the argument, or the device-side sub-step's; false
+// when the device-side loop has ended and the launch is to do nothing
+template <class T>
+__device__ __forceinline__ bool step_dt(const FluidArgs<T> &a, double &dt_d)
+{
+    if (!a.step)
+        return true;
+    if (a.step[STEP_ACTIVE] == 0ull)
+        return false;
+    dt_d = __longlong_as_double((long long)a.step[STEP_DT]);
+    return true;
+}
 
 // position/volume record of particle j
 template <class T, int D>
@@ -270,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) k_init_1st(FluidArgs<typename Prec<P
 {
     using T = typename Prec<P>::T;
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= a.n)
+    if (i >= a.n || !step_dt(a, dt_d))
         return;
     const T dt = T(dt_d);
     T c = a.C[i] + T(0.5) * dt * a.dC[i];
@@ -293,7 +309,7 @@ __global__ void __launch_bounds__(kThreads) k_init_2nd(FluidArgs<typename Prec<P
 {
     using T = typename Prec<P>::T;
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= a.n)
+    if (i >= a.n || !step_dt(a, dt_d))
         return;
     const T dt = T(dt_d);
     Rec4<T> v = a.vel[i];
@@ -345,6 +361,8 @@ template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
 __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
 {
     using T = typename Prec<P>::T;
+    if (!step_dt(a, dt_d)) // grid-uniform: before any barrier
+        return;
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
     extern __shared__ __align__(32) unsigned char staged_lds[];
@@ -520,6 +538,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P
 k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_slot)
 {
     using T = typename Prec<P>::T;
+    if (!step_dt(a, dt_d)) // grid-uniform: before any barrier
+        return;
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
     __shared__ double wave_red[kThreads / 64];
@@ -1505,6 +1525,70 @@ __global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> 
     out[i] = q / (w + T(2.71051e-20));
 }
 
+//---------------------------------------------------------------------------
+// device-side dual-time loop of sphx_advection_step (dambreak.cpp:188-207):
+// AdvectionTimeStepCK's dt from the speed max, then per sub-step
+// AcousticTimeStepCK's dt from the signal max and the loop condition
+// relaxation < advection dt, with the host's arithmetic in double
+//---------------------------------------------------------------------------
+__device__ __forceinline__ double lanes_max(const unsigned long long *lanes)
+{
+    unsigned long long m = 0;
+    for (int k = 0; k < kMaxLanes; ++k)
+        m = m > lanes[k * kMaxStride] ? m : lanes[k * kMaxStride];
+    return from_ordered_bits(m);
+}
+__device__ __forceinline__ void put_double(unsigned long long *slot, double v)
+{
+    *slot = (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ double get_double(const unsigned long long *slot)
+{
+    return __longlong_as_double((long long)*slot);
+}
+
+__global__ void k_step_begin(unsigned long long *scalars, double u_ref, double cfl, double h, double time)
+{
+    if (threadIdx.x != 0)
+        return;
+    unsigned long long *st = scalars + SLOT_STEP;
+    const double m = lanes_max(scalars + SLOT_MAX);
+    // AdvectionTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.cpp:23-26
+    put_double(st + STEP_ADV_DT, cfl * h / (fmax(sqrt(m), u_ref) + 2.71051e-20));
+    put_double(st + STEP_DT, 0.0);
+    put_double(st + STEP_RELAXATION, 0.0);
+    put_double(st + STEP_TIME, time);
+    st[STEP_COUNT] = 0;
+    st[STEP_ACTIVE] = 0;
+    st[STEP_ERROR] = 0;
+}
+
+__global__ void k_substep_begin(unsigned long long *scalars, double cfl, double h)
+{
+    if (threadIdx.x != 0)
+        return;
+    unsigned long long *st = scalars + SLOT_STEP;
+    const double relaxation = get_double(st + STEP_RELAXATION);
+    if (st[STEP_ERROR] != 0 || !(relaxation < get_double(st + STEP_ADV_DT)))
+    {
+        st[STEP_ACTIVE] = 0;
+        return;
+    }
+    // AcousticTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.hpp:30-57
+    const double dt = cfl * h / (lanes_max(scalars + SLOT_MAX) + 2.71051e-20);
+    if (!isfinite(dt) || !(dt > 0.0))
+    {
+        st[STEP_ERROR] = 1;
+        st[STEP_ACTIVE] = 0;
+        return;
+    }
+    put_double(st + STEP_DT, dt);
+    put_double(st + STEP_RELAXATION, relaxation + dt);
+    put_double(st + STEP_TIME, get_double(st + STEP_TIME) + dt);
+    st[STEP_COUNT] += 1;
+    st[STEP_ACTIVE] = 1;
+}
+
 } // namespace
 
 //=============================================================================
@@ -1515,6 +1599,7 @@ static FluidArgs<T> fluid_args(sphx_body *b)
 {
     FluidArgs<T> a{};
     a.n = b->n;
+    a.step = b->ctx->device_step ? b->ctx->scalars_d + SLOT_STEP : nullptr;
     a.posvol = (const Rec4<T> *)b->posvol;
     a.vel = (Rec4<T> *)b->vel;
     for (int k = 0; k < 3; ++k)
@@ -2219,6 +2304,21 @@ static sphx_status build_poly_t(sphx_relation *rel)
 sphx_status build_kernel_poly(sphx_relation *rel)
 {
     return rel->source->P == P_FP64 ? build_poly_t<double>(rel) : build_poly_t<float>(rel);
+}
+
+sphx_status step_begin(sphx_context *ctx, double u_ref, double advection_cfl, double h, double time)
+{
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->scalars_d, u_ref, advection_cfl, h,
+                       time);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+sphx_status substep_begin(sphx_context *ctx, double acoustic_cfl, double h)
+{
+    hipLaunchKernelGGL(k_substep_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->scalars_d, acoustic_cfl, h);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
 }
 
 } // namespace sphx

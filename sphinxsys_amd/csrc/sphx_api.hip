@@ -85,21 +85,35 @@ sphx_status transfer_scalar_values(sphx_body *b, void *dev, double *h, bool writ
     return SPHX_OK;
 }
 
+// events come from a per-context pool (creating two events per launch costs
+// more host time than the small kernels of a step take on the device)
+static hipEvent_t pooled_event(sphx_context *ctx)
+{
+    if (!ctx->event_pool.empty())
+    {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
 ProfileScope::ProfileScope(sphx_context *c, const char *s) : ctx(c), stage(s)
 {
-    if (ctx->profiling && hipEventCreate(&start) == hipSuccess)
+    const bool timed = ctx->profiling && (!ctx->profile_passes_only || std::strncmp(s, "acoustic_pass", 13) == 0);
+    start = timed ? pooled_event(ctx) : nullptr;
+    if (start)
         (void)hipEventRecord(start, ctx->stream);
-    else
-        start = nullptr;
 }
 ProfileScope::~ProfileScope()
 {
     if (!start)
         return;
-    hipEvent_t stop = nullptr;
-    if (hipEventCreate(&stop) != hipSuccess)
+    hipEvent_t stop = pooled_event(ctx);
+    if (!stop)
     {
-        (void)hipEventDestroy(start);
+        ctx->event_pool.push_back(start);
         return;
     }
     (void)hipEventRecord(stop, ctx->stream);
@@ -117,8 +131,8 @@ static void drain_profile(sphx_context *ctx)
             t.first += ms;
             t.second += 1;
         }
-        (void)hipEventDestroy(e.start);
-        (void)hipEventDestroy(e.stop);
+        ctx->event_pool.push_back(e.start);
+        ctx->event_pool.push_back(e.stop);
     }
     ctx->pending.clear();
 }
@@ -400,6 +414,8 @@ sphx_status sphx_context_destroy(sphx_context *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     drain_profile(ctx);
+    for (hipEvent_t e : ctx->event_pool)
+        (void)hipEventDestroy(e);
     (void)hipFree(ctx->scalars_d);
     (void)hipHostFree(ctx->scalars_h);
     (void)hipStreamDestroy(ctx->stream);
@@ -423,7 +439,10 @@ sphx_status sphx_profile_enable(sphx_context *ctx, int enable)
     SPHX_HIP(hipStreamSynchronize(ctx->stream));
     drain_profile(ctx);
     ctx->profile.clear();
+    if (enable < 0 || enable > 2)
+        return arg_error("profile enable is 0, 1 or 2");
     ctx->profiling = enable != 0;
+    ctx->profile_passes_only = enable == 2;
     return SPHX_OK;
 }
 
@@ -1274,8 +1293,10 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     if ((s = compression_summation(inner, walls, n_wall)) || (s = density_regularization(b, cfg->flow_type)) ||
         (s = advection_step_setup(b)))
         return s;
-    double adv_dt = 0.0;
-    if ((s = sphx_advection_time_step(b, cfg->u_ref, cfg->advection_cfl, &adv_dt)))
+    // AdvectionTimeStepCK on the device: the loop below runs without host
+    // round trips, the host reads the loop state once per batch of sub-steps
+    if ((s = reset_max(ctx)) || (s = advection_speed_max(b, ctx->scalars_d + SLOT_MAX)) ||
+        (s = step_begin(ctx, cfg->u_ref, cfg->advection_cfl, b->h, *time)))
         return s;
     // dambreak.cpp:192-193
     if (cfg->surface_indication && (s = free_surface_indication(inner, walls, n_wall)))
@@ -1284,38 +1305,57 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     if (corrected && (s = linear_correction_matrix(inner, walls, n_wall, cfg->correction_alpha)))
         return s;
     // AcousticTimeStepCK of the first sub-step from the current state
-    double dt = 0.0;
-    if ((s = sphx_acoustic_time_step(b, cfg->acoustic_cfl, &dt)))
+    if ((s = reset_max(ctx)) || (s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX)))
         return s;
-    double relaxation_time = 0.0;
-    uint32_t count = 0;
-    while (relaxation_time < adv_dt)
+    struct DeviceStep
     {
-        if (corrected)
-            s = acoustic_1st_half_corrected(inner, walls, n_wall, cfg->riemann, dt, true);
-        else
-            s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, dt, true);
-        if (s || (s = reset_max(ctx)))
+        sphx_context *c;
+        explicit DeviceStep(sphx_context *c) : c(c) { c->device_step = true; }
+        ~DeviceStep() { c->device_step = false; }
+    } device_step(ctx);
+    // dambreak.cpp:196-207: while (relaxation < advection dt) { dt = acoustic dt; 1st
+    // half; 2nd half; relaxation += dt }.  Each sub-step's dt, the loop condition
+    // and the time are evaluated on the device (k_substep_begin); launches past
+    // the end of the loop return at once.  A batch is as long as the previous
+    // step's loop; when it was too short the loop goes on one sub-step at a time.
+    const unsigned long long *st = ctx->scalars_h + SLOT_STEP;
+    uint32_t batch = ctx->last_substeps ? ctx->last_substeps : 1u;
+    while (true)
+    {
+        for (uint32_t k = 0; k < batch; ++k)
+        {
+            if ((s = substep_begin(ctx, cfg->acoustic_cfl, b->h)))
+                return s;
+            if (corrected)
+                s = acoustic_1st_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, true);
+            else
+                s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, 0.0, true);
+            if (s || (s = reset_max(ctx)))
+                return s;
+            if (corrected)
+                s = acoustic_2nd_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, false, true);
+            else
+                s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, 0.0, false, true);
+            if (s)
+                return s;
+        }
+        if ((s = read_scalars(ctx, SLOT_STEP, STEP_SLOTS)))
             return s;
-        if (corrected)
-            s = acoustic_2nd_half_corrected(inner, walls, n_wall, cfg->riemann, dt, false, true);
-        else
-            s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, dt, false, true);
-        if (s)
-            return s;
-        relaxation_time += dt;
-        *time += dt;
-        ++count;
-        double m = 0.0;
-        if ((s = read_max(ctx, &m)))
-            return s;
-        dt = cfg->acoustic_cfl * b->h / (m + 2.71051e-20);
-        if (!std::isfinite(dt) || !(dt > 0.0))
+        if (st[STEP_ERROR])
         {
             set_error("non-finite acoustic time step: the fluid state diverged");
             return SPHX_E_NONFINITE;
         }
+        double relaxation, adv_dt;
+        std::memcpy(&relaxation, st + STEP_RELAXATION, sizeof(double));
+        std::memcpy(&adv_dt, st + STEP_ADV_DT, sizeof(double));
+        if (!(relaxation < adv_dt))
+            break;
+        batch = 1;
     }
+    const uint32_t count = (uint32_t)st[STEP_COUNT];
+    std::memcpy(time, st + STEP_TIME, sizeof(double));
+    ctx->last_substeps = count;
     if ((s = sphx_update_particle_position(b)))
         return s;
     if (cfg->sort_now && (s = sphx_particle_sort(b, cll)))

@@ -43,7 +43,21 @@ enum ScalarSlot
     SLOT_DT = 3,         // double: acoustic dt of the current sub-step
     SLOT_NEEDED = 4,     // u64: neighbour count beyond the slice width
     SLOT_SUM_BASE = 8,   // double partial sums region start (reduce_sum final)
+    SLOT_STEP = 16,      // device-side sub-step control of sphx_advection_step (StepSlot offsets)
     SLOT_MAX = 512       // kMaxLanes ordered-bits max accumulators, kMaxStride apart
+};
+// offsets from SLOT_STEP: the dual-time loop state kept on the device so that
+// the acoustic sub-steps of an advection step run without host round trips
+enum StepSlot
+{
+    STEP_ADV_DT = 0,     // double: advection dt of the step
+    STEP_DT = 1,         // double: acoustic dt of the current sub-step
+    STEP_RELAXATION = 2, // double: relaxation time so far
+    STEP_TIME = 3,       // double: physical time
+    STEP_COUNT = 4,      // u64: sub-steps run
+    STEP_ACTIVE = 5,     // u64: 1 while the current sub-step runs
+    STEP_ERROR = 6,      // u64: 1 when a non-finite acoustic dt stopped the loop
+    STEP_SLOTS = 7
 };
 } // namespace sphx
 
@@ -54,16 +68,21 @@ struct sphx_context
     size_t max_lds = 65536; // shared memory per workgroup of the device
     // HIP-event kernel timing (sphx_profile_*)
     bool profiling = false;
+    bool profile_passes_only = false; // time only the acoustic pass launches
     struct PendingEvent
     {
         std::string stage;
         hipEvent_t start, stop;
     };
     std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool; // recorded events ready for reuse
     std::map<std::string, std::pair<double, uint64_t>> profile; // stage -> (ms, launches)
     unsigned long long *scalars_d = nullptr; // kScalarSlots u64 on the device
     unsigned long long *scalars_h = nullptr; // pinned mirror
     sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp, cll_runs;
+    // while set, the acoustic kernels take dt and their run flag from the SLOT_STEP slots
+    bool device_step = false;
+    uint32_t last_substeps = 0; // sub-steps of the previous sphx_advection_step
 };
 
 struct sphx_body
@@ -194,6 +213,10 @@ sphx_status compression_summation(sphx_relation *inner, sphx_relation *const *c,
 sphx_status acoustic_1st_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                               double dt, bool fuse_init_2nd);
 // run_init: launch the 2nd half's initialize (false when pass 1 fused it);
+// device-side dual-time loop: the step state from the advection speed max in
+// the SLOT_MAX lanes, then per sub-step its dt from the signal max there
+sphx_status step_begin(sphx_context *ctx, double u_ref, double advection_cfl, double h, double time);
+sphx_status substep_begin(sphx_context *ctx, double acoustic_cfl, double h);
 // fuse_cfl: max-reduce the next sub-step's signal speed into the SLOT_MAX lanes
 sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                               double dt, bool run_init, bool fuse_cfl);
