@@ -287,8 +287,12 @@ sphx_status sphx_interpolate_contact(sphx_relation *observer_contact, int field,
  * [free-surface indication] -> [linear correction matrix] ->
  * acoustic sub-steps (dt, 1st half, 2nd half) until their sum reaches the
  * advection dt -> position update -> [sort when sort_now] -> cell linked
- * list -> relations.  The acoustic time step size stays on the device.
- * On return *physical_time has advanced and *acoustic_steps holds the number
+ * list -> relations.  The advection dt, each sub-step's acoustic dt, the loop
+ * condition and the physical time are evaluated on the device in double (the
+ * reference's host arithmetic); the host reads the loop state back once per
+ * batch of sub-steps (a batch is as long as the previous call's loop).  A
+ * non-finite acoustic dt ends the call with SPHX_E_NONFINITE before the
+ * position update.  On return *physical_time has advanced and *acoustic_steps holds the number
  * of sub-steps taken. No reference counterpart: it is the case-file loop body
  * executed behind one call. */
 typedef struct sphx_advection_config {
