@@ -338,6 +338,65 @@ __global__ void k_run_place(const uint32_t *keys, const uint32_t *ex, const uint
     }
 }
 
+// the runs of a small set (at most kSmallRuns) in one workgroup: sorted by
+// (cell, run number) with a bitonic network in LDS, which is the order of the
+// stable sort, then each run's output offset from the exclusive sum of the run
+// lengths in that order
+constexpr uint32_t kSmallRuns = 4096;
+constexpr uint32_t kSmallThreads = 1024;
+__global__ void __launch_bounds__(kSmallThreads) k_runs_small(const uint32_t *run_key, const uint32_t *run_start,
+                                                              uint32_t nruns, uint32_t n, uint32_t *run_off)
+{
+    __shared__ unsigned long long v[kSmallRuns];
+    __shared__ uint32_t wave_tot[kSmallThreads / 64];
+    const uint32_t t = threadIdx.x;
+    uint32_t p = 1;
+    while (p < nruns)
+        p <<= 1;
+    for (uint32_t q = t; q < p; q += kSmallThreads)
+        v[q] = q < nruns ? ((unsigned long long)run_key[q] << 32) | q : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= p; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1)
+        {
+            for (uint32_t a = t; a < p; a += kSmallThreads)
+            {
+                const uint32_t b = a ^ j;
+                if (b > a)
+                {
+                    const unsigned long long x = v[a], y = v[b];
+                    if ((x > y) == ((a & k) == 0))
+                    {
+                        v[a] = y;
+                        v[b] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // thread t owns sorted positions [t * per, (t + 1) * per)
+    const uint32_t per = (nruns + kSmallThreads - 1) / kSmallThreads;
+    const uint32_t lo = min(nruns, t * per), hi = min(nruns, lo + per);
+    auto length = [&](uint32_t r) { return (r + 1u < nruns ? run_start[r + 1u] : n) - run_start[r]; };
+    uint32_t sum = 0;
+    for (uint32_t q = lo; q < hi; ++q)
+        sum += length((uint32_t)(v[q] & 0xffffffffu));
+    const uint32_t inc = wave_inclusive_scan(sum);
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    if (lane == 63u)
+        wave_tot[wave] = inc;
+    __syncthreads();
+    uint32_t base = inc - sum;
+    for (uint32_t w = 0; w < wave; ++w)
+        base += wave_tot[w];
+    for (uint32_t q = lo; q < hi; ++q)
+    {
+        const uint32_t r = (uint32_t)(v[q] & 0xffffffffu);
+        run_off[r] = base;
+        base += length(r);
+    }
+}
+
 template <int D>
 __global__ void __launch_bounds__(kThreads) k_morton_keys(PosPtrs pos, uint32_t n, MeshParams m, uint32_t *keys,
                                                           uint32_t *vals)
@@ -1302,13 +1361,22 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
         return s;
     hipLaunchKernelGGL(k_run_emit, g, blk, 0, ctx->stream, cll->keys, ex, n, run_key, run_idx, run_start);
     SPHX_HIP(hipGetLastError());
-    if ((s = radix_sort_pairs(ctx, run_key, run_idx, nruns, cll->key_bits)))
-        return s;
-    const dim3 gr(grid_for(nruns, 256));
-    hipLaunchKernelGGL(k_run_len, gr, blk, 0, ctx->stream, run_idx, run_start, nruns, n, len);
-    if ((s = exclusive_scan_u32(ctx, len, len, nruns, nullptr)))
-        return s;
-    hipLaunchKernelGGL(k_run_off, gr, blk, 0, ctx->stream, run_idx, len, nruns, run_off);
+    if (nruns <= kSmallRuns)
+    {
+        // one launch instead of the radix passes (small scenes are bound by launch counts)
+        hipLaunchKernelGGL(k_runs_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, run_key, run_start, nruns, n,
+                           run_off);
+    }
+    else
+    {
+        if ((s = radix_sort_pairs(ctx, run_key, run_idx, nruns, cll->key_bits)))
+            return s;
+        const dim3 gr(grid_for(nruns, 256));
+        hipLaunchKernelGGL(k_run_len, gr, blk, 0, ctx->stream, run_idx, run_start, nruns, n, len);
+        if ((s = exclusive_scan_u32(ctx, len, len, nruns, nullptr)))
+            return s;
+        hipLaunchKernelGGL(k_run_off, gr, blk, 0, ctx->stream, run_idx, len, nruns, run_off);
+    }
     hipLaunchKernelGGL(k_run_place, g, blk, 0, ctx->stream, cll->keys, ex, run_start, run_off, n,
                        cll->particle_index);
     SPHX_HIP(hipGetLastError());
