@@ -1470,8 +1470,12 @@ sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long
     if (n == 0)
         return SPHX_OK;
     const uint32_t nchunks = (n + 63u) / 64u;
-    hipLaunchKernelGGL(k_chunk_stride, dim3(grid_for(nchunks + 1, 256)), dim3(256), 0, ctx->stream,
-                       rel->chunk_offset, nchunks, kmax * 64u);
+    if (rel->chunk_kmax != kmax) // the chunk offsets depend on the slice width only
+    {
+        hipLaunchKernelGGL(k_chunk_stride, dim3(grid_for(nchunks + 1, 256)), dim3(256), 0, ctx->stream,
+                           rel->chunk_offset, nchunks, kmax * 64u);
+        rel->chunk_kmax = kmax;
+    }
     BuildArgs b;
     b.r = relation_args(rel);
     b.kmax = kmax;

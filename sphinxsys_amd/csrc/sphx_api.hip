@@ -921,10 +921,7 @@ sphx_status sphx_update_relation(sphx_relation *r)
     r->slice_total = nchunks * kmax * 64;
     if ((s = relation_stage(r)))
         return s;
-    // CSR offsets of the reference layout (NeighborSize scan), for reads
-    if ((s = exclusive_scan_u32(ctx, r->neighbor_size, r->particle_offset, n + 1, nullptr)))
-        return s;
-    r->csr_valid = false;
+    r->csr_valid = false; // the CSR view (offsets and indices) is built when read
     r->built = true;
     r->builds++;
     r->built_version = r->source->config_version;
@@ -945,6 +942,10 @@ sphx_status sphx_relation_read(sphx_relation *r, uint32_t *size, uint32_t *offse
     size_t n = r->source->n;
     if (!r->csr_valid)
     {
+        // CSR offsets of the reference layout (NeighborSize scan), then the indices
+        sphx_status st = exclusive_scan_u32(ctx, r->neighbor_size, r->particle_offset, n + 1, nullptr);
+        if (st)
+            return st;
         uint32_t t = 0;
         SPHX_HIP(hipMemcpyAsync(&t, r->particle_offset + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         SPHX_HIP(hipStreamSynchronize(s));
@@ -957,8 +958,7 @@ sphx_status sphx_relation_read(sphx_relation *r, uint32_t *size, uint32_t *offse
             r->capacity = std::max<size_t>(t, 64);
             SPHX_HIP(hipMalloc(&r->neighbor_index, r->capacity * sizeof(uint32_t)));
         }
-        sphx_status st = relation_to_csr(r);
-        if (st)
+        if ((st = relation_to_csr(r)))
             return st;
         r->csr_valid = true;
     }

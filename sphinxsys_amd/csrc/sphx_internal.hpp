@@ -162,6 +162,7 @@ struct sphx_relation
     uint32_t *slice = nullptr;
     size_t slice_capacity = 0, slice_total = 0;
     uint32_t kmax = 0;                    // rows per 64-particle chunk of the slices
+    uint32_t chunk_kmax = 0;              // slice width chunk_offset was last written for
     bool csr_valid = false;
     // LDS staging of the inner relation: per block of kStageBlock source
     // particles, the set of distinct neighbours (stage_keys[b * stage_cap + r],
