@@ -130,6 +130,32 @@ __global__ void __launch_bounds__(kThreads) k_scan_apply(const uint32_t *in, uin
         }
 }
 
+// the whole scan of at most one tile in one launch; the total to *total
+__global__ void __launch_bounds__(kThreads) k_scan_one_tile(const uint32_t *in, uint32_t *out, size_t n,
+                                                            unsigned long long *total)
+{
+    const size_t base = (size_t)threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(s, &tot); // its barriers order the reads before the writes
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (base + k < n)
+        {
+            out[base + k] = run;
+            run += v[k];
+        }
+    if (threadIdx.x == 0)
+        *total = tot;
+}
+
 //---------------------------------------------------------------------------
 // LSD radix sort of (key, value) pairs, 4-bit digits, stable.
 //---------------------------------------------------------------------------
@@ -1261,9 +1287,14 @@ sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *
         return s;
     uint32_t *sums = (uint32_t *)ctx->scan_blocks.ptr;
     unsigned long long *tot = ctx->scalars_d + SLOT_TOTAL;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kThreads), 0, ctx->stream, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, sums, ntiles, tot);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kThreads), 0, ctx->stream, in, out, n, sums);
+    if (ntiles == 1)
+        hipLaunchKernelGGL(k_scan_one_tile, dim3(1), dim3(kThreads), 0, ctx->stream, in, out, n, tot);
+    else
+    {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kThreads), 0, ctx->stream, in, n, sums);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, sums, ntiles, tot);
+        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kThreads), 0, ctx->stream, in, out, n, sums);
+    }
     SPHX_HIP(hipGetLastError());
     if (total_host)
     {

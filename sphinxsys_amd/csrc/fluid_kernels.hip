@@ -1531,11 +1531,15 @@ __global__ void k_interpolate(const Rec4<T> *obs, uint32_t n_obs, const Rec4<T> 
 // AcousticTimeStepCK's dt from the signal max and the loop condition
 // relaxation < advection dt, with the host's arithmetic in double
 //---------------------------------------------------------------------------
-__device__ __forceinline__ double lanes_max(const unsigned long long *lanes)
+// the max of the accumulator lanes, which are then reset for the next reduction
+__device__ __forceinline__ double lanes_max_reset(unsigned long long *lanes)
 {
     unsigned long long m = 0;
     for (int k = 0; k < kMaxLanes; ++k)
+    {
         m = m > lanes[k * kMaxStride] ? m : lanes[k * kMaxStride];
+        lanes[k * kMaxStride] = kOrderedLowest;
+    }
     return from_ordered_bits(m);
 }
 __device__ __forceinline__ void put_double(unsigned long long *slot, double v)
@@ -1552,7 +1556,7 @@ __global__ void k_step_begin(unsigned long long *scalars, double u_ref, double c
     if (threadIdx.x != 0)
         return;
     unsigned long long *st = scalars + SLOT_STEP;
-    const double m = lanes_max(scalars + SLOT_MAX);
+    const double m = lanes_max_reset(scalars + SLOT_MAX);
     // AdvectionTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.cpp:23-26
     put_double(st + STEP_ADV_DT, cfl * h / (fmax(sqrt(m), u_ref) + 2.71051e-20));
     put_double(st + STEP_DT, 0.0);
@@ -1575,7 +1579,7 @@ __global__ void k_substep_begin(unsigned long long *scalars, double cfl, double 
         return;
     }
     // AcousticTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.hpp:30-57
-    const double dt = cfl * h / (lanes_max(scalars + SLOT_MAX) + 2.71051e-20);
+    const double dt = cfl * h / (lanes_max_reset(scalars + SLOT_MAX) + 2.71051e-20);
     if (!isfinite(dt) || !(dt > 0.0))
     {
         st[STEP_ERROR] = 1;

@@ -164,7 +164,6 @@ sphx_status alloc(sphx_body *b, void **p, size_t bytes)
     return SPHX_OK;
 }
 
-constexpr unsigned long long kOrderedLowest = 0x0010000000000000ull; // ordered_bits(-DBL_MAX)
 
 // host <-> device transfer of one field with layout/precision conversion
 template <class T>
@@ -1304,8 +1303,9 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     const bool corrected = cfg->correction == SPHX_CORRECTION_LINEAR;
     if (corrected && (s = linear_correction_matrix(inner, walls, n_wall, cfg->correction_alpha)))
         return s;
-    // AcousticTimeStepCK of the first sub-step from the current state
-    if ((s = reset_max(ctx)) || (s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX)))
+    // AcousticTimeStepCK of the first sub-step from the current state (k_step_begin
+    // left the max lanes reset)
+    if ((s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX)))
         return s;
     struct DeviceStep
     {
@@ -1326,11 +1326,12 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
         {
             if ((s = substep_begin(ctx, cfg->acoustic_cfl, b->h)))
                 return s;
+            // k_substep_begin read the signal max and reset the lanes for pass 2
             if (corrected)
                 s = acoustic_1st_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, true);
             else
                 s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, 0.0, true);
-            if (s || (s = reset_max(ctx)))
+            if (s)
                 return s;
             if (corrected)
                 s = acoustic_2nd_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, false, true);

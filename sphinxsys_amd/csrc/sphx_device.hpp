@@ -109,6 +109,7 @@ __host__ __device__ inline double from_ordered_bits(unsigned long long u)
 // Device-scope atomics on one address serialise in memory, so each workgroup
 // uses one of kMaxLanes accumulators, kMaxStride u64 (one 128-B line) apart,
 // and the reader takes the maximum of the lanes.
+constexpr unsigned long long kOrderedLowest = 0x0010000000000000ull; // ordered_bits(-DBL_MAX)
 constexpr int kMaxLanes = 32;
 constexpr int kMaxStride = 16;
 __device__ __forceinline__ void atomic_max_spread(unsigned long long *base, double v)
