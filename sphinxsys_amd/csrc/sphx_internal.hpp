@@ -40,7 +40,6 @@ constexpr unsigned long long kTooDense = 1ull << 40;
 enum ScalarSlot
 {
     SLOT_TOTAL = 2,      // u32 totals returned by scans (stored as u64)
-    SLOT_DT = 3,         // double: acoustic dt of the current sub-step
     SLOT_NEEDED = 4,     // u64: neighbour count beyond the slice width
     SLOT_SUM_BASE = 8,   // double partial sums region start (reduce_sum final)
     SLOT_STEP = 16,      // device-side sub-step control of sphx_advection_step (StepSlot offsets)
