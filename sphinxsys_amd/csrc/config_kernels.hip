@@ -1423,7 +1423,11 @@ sphx_status cell_near_flags(sphx_cell_linked_list *cll)
     if (cll->near && cll->near_build == cll->build_count)
         return SPHX_OK;
     if (!cll->near)
-        SPHX_HIP(hipMalloc(&cll->near, (size_t)cll->ncells));
+    {
+        sphx_status s = dev_alloc(ctx->stream, (void **)&cll->near, (size_t)cll->ncells);
+        if (s)
+            return s;
+    }
     const uint32_t nc = (uint32_t)cll->ncells;
     if (cll->body->D == 2)
         hipLaunchKernelGGL(k_near_flags<2>, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,

@@ -2299,7 +2299,11 @@ static sphx_status build_poly_t(sphx_relation *rel)
                            (f0 * f1 * f3) / dl2 * w[2] + (f0 * f1 * f2) / dl3 * w[3];
     kp.w0 = T((D == 2 ? src_inv_h2 : src_inv_h3) * dimfac * interp0);
     if (!rel->poly)
-        SPHX_HIP(hipMalloc(&rel->poly, sizeof(KernelPoly<T>)));
+    {
+        sphx_status s = dev_alloc(src->ctx->stream, (void **)&rel->poly, sizeof(KernelPoly<T>));
+        if (s)
+            return s;
+    }
     SPHX_HIP(hipMemcpyAsync(rel->poly, &kp, sizeof(kp), hipMemcpyHostToDevice, src->ctx->stream));
     SPHX_HIP(hipStreamSynchronize(src->ctx->stream));
     return SPHX_OK;

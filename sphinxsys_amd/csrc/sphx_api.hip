@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 
 namespace sphx
 {
@@ -31,27 +32,92 @@ static sphx_status state_error(const std::string &msg)
     return SPHX_E_STATE;
 }
 
+// blocks released by destroyed objects, kept for reuse: a block serves a
+// request of up to its size and at least half of it
+namespace
+{
+std::mutex g_mem_mutex;
+std::map<int, std::multimap<size_t, void *>> g_free_blocks; // device -> (bytes, block)
+std::map<void *, std::pair<int, size_t>> g_block_size;      // block -> (device, bytes)
+} // namespace
+
+sphx_status dev_alloc(hipStream_t s, void **p, size_t bytes)
+{
+    bytes = std::max<size_t>(bytes, 256);
+    int dev = 0;
+    SPHX_HIP(hipStreamGetDevice(s, &dev));
+    {
+        std::lock_guard<std::mutex> lock(g_mem_mutex);
+        auto &blocks = g_free_blocks[dev];
+        auto it = blocks.lower_bound(bytes);
+        if (it != blocks.end() && it->first / 2 <= bytes)
+        {
+            *p = it->second;
+            blocks.erase(it);
+            return SPHX_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory)
+    {
+        // return the kept blocks of this device and try once more
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lock(g_mem_mutex);
+        for (auto &kv : g_free_blocks[dev])
+        {
+            (void)hipFree(kv.second);
+            g_block_size.erase(kv.second);
+        }
+        g_free_blocks[dev].clear();
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "hipMalloc");
+    std::lock_guard<std::mutex> lock(g_mem_mutex);
+    g_block_size[*p] = {dev, bytes};
+    return SPHX_OK;
+}
+void dev_free(hipStream_t s, void *p)
+{
+    if (!p)
+        return;
+    // the block may still be read by kernels queued on its context's stream
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lock(g_mem_mutex);
+    auto it = g_block_size.find(p);
+    if (it != g_block_size.end())
+        g_free_blocks[it->second.first].emplace(it->second.second, p);
+}
+
+// streams and pinned scalar mirrors of destroyed contexts, reused by the next
+// ones (destroying them can wait for the whole device)
+namespace
+{
+std::mutex g_reuse_mutex;
+std::map<int, std::vector<hipStream_t>> g_free_streams;
+std::vector<unsigned long long *> g_free_mirrors;
+} // namespace
+
 sphx_status Scratch::reserve(size_t n)
 {
     if (n <= bytes)
         return SPHX_OK;
-    if (ptr)
-    {
-        // the old buffer may still be in use by queued kernels
-        SPHX_HIP(hipDeviceSynchronize());
-        SPHX_HIP(hipFree(ptr));
-        ptr = nullptr;
-        bytes = 0;
-    }
+    // the old buffer is released after the kernels already queued on the stream
+    dev_free(stream, ptr);
+    ptr = nullptr;
+    bytes = 0;
     size_t want = n + n / 4;
-    SPHX_HIP(hipMalloc(&ptr, want));
+    sphx_status s = dev_alloc(stream, &ptr, want);
+    if (s)
+        return s;
     bytes = want;
     return SPHX_OK;
 }
-Scratch::~Scratch()
+void Scratch::release()
 {
-    if (ptr)
-        (void)hipFree(ptr);
+    dev_free(stream, ptr);
+    ptr = nullptr;
+    bytes = 0;
 }
 
 sphx_status read_scalars(sphx_context *ctx, int first, int count)
@@ -159,7 +225,9 @@ sphx_status fill(sphx_context *ctx, void *p, size_t n, T v)
 
 sphx_status alloc(sphx_body *b, void **p, size_t bytes)
 {
-    SPHX_HIP(hipMalloc(p, std::max<size_t>(bytes, 64)));
+    sphx_status s = dev_alloc(b->ctx->stream, p, bytes);
+    if (s)
+        return s;
     b->allocations.push_back(*p);
     return SPHX_OK;
 }
@@ -396,12 +464,34 @@ sphx_status sphx_context_create(int device, sphx_context **out)
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         ctx->max_lds = (size_t)lds;
-    SPHX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    SPHX_HIP(hipMalloc(&ctx->scalars_d, kScalarSlots * sizeof(unsigned long long)));
-    SPHX_HIP(hipHostMalloc(&ctx->scalars_h, kScalarSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    sphx_status st;
+    {
+        std::lock_guard<std::mutex> lock(g_reuse_mutex);
+        auto &streams = g_free_streams[device];
+        if (!streams.empty())
+        {
+            ctx->stream = streams.back();
+            streams.pop_back();
+        }
+        if (!g_free_mirrors.empty())
+        {
+            ctx->scalars_h = g_free_mirrors.back();
+            g_free_mirrors.pop_back();
+        }
+    }
+    if (!ctx->stream)
+        SPHX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (Scratch *sc : {&ctx->sort_keys, &ctx->sort_vals, &ctx->sort_hist, &ctx->scan_blocks, &ctx->reduce_parts,
+                        &ctx->gather_tmp, &ctx->cll_runs})
+        sc->stream = ctx->stream;
+    if (!ctx->scalars_h)
+        SPHX_HIP(hipHostMalloc(&ctx->scalars_h, kScalarSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    if ((st = dev_alloc(ctx->stream, (void **)&ctx->scalars_d, kScalarSlots * sizeof(unsigned long long))))
+        return st;
     std::memset(ctx->scalars_h, 0, kScalarSlots * sizeof(unsigned long long));
-    SPHX_HIP(hipMemcpy(ctx->scalars_d, ctx->scalars_h, kScalarSlots * sizeof(unsigned long long),
-                       hipMemcpyHostToDevice));
+    SPHX_HIP(hipMemcpyAsync(ctx->scalars_d, ctx->scalars_h, kScalarSlots * sizeof(unsigned long long),
+                            hipMemcpyHostToDevice, ctx->stream));
+    SPHX_HIP(hipStreamSynchronize(ctx->stream));
     *out = ctx.release();
     return SPHX_OK;
 }
@@ -415,9 +505,16 @@ sphx_status sphx_context_destroy(sphx_context *ctx)
     drain_profile(ctx);
     for (hipEvent_t e : ctx->event_pool)
         (void)hipEventDestroy(e);
-    (void)hipFree(ctx->scalars_d);
-    (void)hipHostFree(ctx->scalars_h);
-    (void)hipStreamDestroy(ctx->stream);
+    for (Scratch *sc : {&ctx->sort_keys, &ctx->sort_vals, &ctx->sort_hist, &ctx->scan_blocks, &ctx->reduce_parts,
+                        &ctx->gather_tmp, &ctx->cll_runs})
+        sc->release();
+    dev_free(ctx->stream, ctx->scalars_d);
+    (void)hipStreamSynchronize(ctx->stream);
+    {
+        std::lock_guard<std::mutex> lock(g_reuse_mutex);
+        g_free_streams[ctx->device].push_back(ctx->stream);
+        g_free_mirrors.push_back(ctx->scalars_h);
+    }
     delete ctx;
     return SPHX_OK;
 }
@@ -582,9 +679,9 @@ sphx_status sphx_body_destroy(sphx_body *b)
 {
     if (!b)
         return SPHX_OK;
-    (void)hipStreamSynchronize(b->ctx->stream);
     for (void *p : b->allocations)
-        (void)hipFree(p);
+        dev_free(b->ctx->stream, p);
+    (void)hipStreamSynchronize(b->ctx->stream);
     delete b;
     return SPHX_OK;
 }
@@ -688,9 +785,12 @@ sphx_status sphx_cell_linked_list_create(sphx_body *b, const double *lo, const d
         ++bits;
     c->key_bits = bits;
     SPHX_HIP(hipSetDevice(b->ctx->device));
-    SPHX_HIP(hipMalloc(&c->cell_offset, (ncells + 1) * sizeof(uint32_t)));
-    SPHX_HIP(hipMalloc(&c->particle_index, std::max<size_t>(b->n, 1) * sizeof(uint32_t)));
-    SPHX_HIP(hipMalloc(&c->keys, std::max<size_t>(b->n, 1) * sizeof(uint32_t)));
+    hipStream_t s = b->ctx->stream;
+    sphx_status st;
+    if ((st = dev_alloc(s, (void **)&c->cell_offset, (ncells + 1) * sizeof(uint32_t))) ||
+        (st = dev_alloc(s, (void **)&c->particle_index, std::max<size_t>(b->n, 1) * sizeof(uint32_t))) ||
+        (st = dev_alloc(s, (void **)&c->keys, std::max<size_t>(b->n, 1) * sizeof(uint32_t))))
+        return st;
     *out = c.release();
     return SPHX_OK;
 }
@@ -699,12 +799,10 @@ sphx_status sphx_cell_linked_list_destroy(sphx_cell_linked_list *c)
 {
     if (!c)
         return SPHX_OK;
-    (void)hipStreamSynchronize(c->body->ctx->stream);
-    (void)hipFree(c->cell_offset);
-    (void)hipFree(c->particle_index);
-    (void)hipFree(c->keys);
-    if (c->near)
-        (void)hipFree(c->near);
+    hipStream_t s = c->body->ctx->stream;
+    for (void *p : {(void *)c->cell_offset, (void *)c->particle_index, (void *)c->keys, (void *)c->near})
+        dev_free(s, p);
+    (void)hipStreamSynchronize(s);
     delete c;
     return SPHX_OK;
 }
@@ -780,9 +878,14 @@ static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_lin
     r->src_inv_h = 1.0 / src->h;
     r->inv_h = 1.0 / std::max(src->h, tar->h);
     SPHX_HIP(hipSetDevice(src->ctx->device));
-    SPHX_HIP(hipMalloc(&r->neighbor_size, (size_t)(src->n + 1) * sizeof(uint32_t)));
-    SPHX_HIP(hipMalloc(&r->particle_offset, (size_t)(src->n + 1) * sizeof(uint32_t)));
-    SPHX_HIP(hipMalloc(&r->chunk_offset, (size_t)((src->n + 63) / 64 + 1) * sizeof(uint32_t)));
+    hipStream_t stream = src->ctx->stream;
+    {
+        sphx_status st;
+        if ((st = dev_alloc(stream, (void **)&r->neighbor_size, (size_t)(src->n + 1) * sizeof(uint32_t))) ||
+            (st = dev_alloc(stream, (void **)&r->particle_offset, (size_t)(src->n + 1) * sizeof(uint32_t))) ||
+            (st = dev_alloc(stream, (void **)&r->chunk_offset, (size_t)((src->n + 63) / 64 + 1) * sizeof(uint32_t))))
+            return st;
+    }
     SPHX_HIP(hipMemsetAsync(r->neighbor_size, 0, (size_t)(src->n + 1) * sizeof(uint32_t), src->ctx->stream));
     if (inner && src->desc.kind == SPHX_BODY_FLUID && src->n > 0)
     {
@@ -796,8 +899,10 @@ static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_lin
             const size_t nblocks = (src->n + kStageBlock - 1) / kStageBlock;
             r->staged = true;
             r->stage_cap = (uint32_t)cap;
-            SPHX_HIP(hipMalloc(&r->stage_count, nblocks * sizeof(uint32_t)));
-            SPHX_HIP(hipMalloc(&r->stage_keys, nblocks * cap * sizeof(uint32_t)));
+            sphx_status st;
+            if ((st = dev_alloc(stream, (void **)&r->stage_count, nblocks * sizeof(uint32_t))) ||
+                (st = dev_alloc(stream, (void **)&r->stage_keys, nblocks * cap * sizeof(uint32_t))))
+                return st;
         }
     }
     sphx_status s = build_kernel_poly(r.get());
@@ -828,22 +933,12 @@ sphx_status sphx_relation_destroy(sphx_relation *r)
 {
     if (!r)
         return SPHX_OK;
-    (void)hipStreamSynchronize(r->source->ctx->stream);
-    (void)hipFree(r->neighbor_size);
-    (void)hipFree(r->particle_offset);
-    if (r->neighbor_index)
-        (void)hipFree(r->neighbor_index);
-    (void)hipFree(r->chunk_offset);
-    if (r->slice)
-        (void)hipFree(r->slice);
-    if (r->stage_count)
-        (void)hipFree(r->stage_count);
-    if (r->stage_keys)
-        (void)hipFree(r->stage_keys);
-    if (r->local_slice)
-        (void)hipFree(r->local_slice);
-    if (r->poly)
-        (void)hipFree(r->poly);
+    hipStream_t s = r->source->ctx->stream;
+    for (void *p : {(void *)r->neighbor_size, (void *)r->particle_offset, (void *)r->neighbor_index,
+                    (void *)r->chunk_offset, (void *)r->slice, (void *)r->stage_count, (void *)r->stage_keys,
+                    (void *)r->local_slice, (void *)r->poly})
+        dev_free(s, p);
+    (void)hipStreamSynchronize(s);
     delete r;
     return SPHX_OK;
 }
@@ -877,16 +972,16 @@ sphx_status sphx_update_relation(sphx_relation *r)
         }
         if (need > r->slice_capacity || !r->slice)
         {
-            SPHX_HIP(hipStreamSynchronize(ctx->stream));
-            if (r->slice)
-                SPHX_HIP(hipFree(r->slice));
+            // released after the kernels already queued on the stream
+            dev_free(ctx->stream, r->slice);
             r->slice = nullptr;
-            if (r->local_slice)
-                SPHX_HIP(hipFree(r->local_slice));
+            dev_free(ctx->stream, r->local_slice);
             r->local_slice = nullptr;
-            SPHX_HIP(hipMalloc(&r->slice, need * sizeof(uint32_t)));
-            if (r->staged)
-                SPHX_HIP(hipMalloc(&r->local_slice, need * sizeof(uint16_t)));
+            r->slice_capacity = 0;
+            if ((s = dev_alloc(ctx->stream, (void **)&r->slice, need * sizeof(uint32_t))))
+                return s;
+            if (r->staged && (s = dev_alloc(ctx->stream, (void **)&r->local_slice, need * sizeof(uint16_t))))
+                return s;
             r->slice_capacity = need;
         }
         if ((s = fill_u64(ctx, ctx->scalars_d + SLOT_NEEDED, 1, 0ull)) ||
@@ -951,11 +1046,13 @@ sphx_status sphx_relation_read(sphx_relation *r, uint32_t *size, uint32_t *offse
         r->total = t;
         if (t > r->capacity || !r->neighbor_index)
         {
-            if (r->neighbor_index)
-                SPHX_HIP(hipFree(r->neighbor_index));
+            dev_free(s, r->neighbor_index);
             r->neighbor_index = nullptr;
-            r->capacity = std::max<size_t>(t, 64);
-            SPHX_HIP(hipMalloc(&r->neighbor_index, r->capacity * sizeof(uint32_t)));
+            r->capacity = 0;
+            const size_t cap = std::max<size_t>(t, 64);
+            if ((st = dev_alloc(s, (void **)&r->neighbor_index, cap * sizeof(uint32_t))))
+                return st;
+            r->capacity = cap;
         }
         if ((st = relation_to_csr(r)))
             return st;

@@ -21,13 +21,24 @@ sphx_status hip_fail(hipError_t e, const char *what);
             return ::sphx::hip_fail(e_, #call);       \
     } while (0)
 
-// grow-only device scratch owned by a context
+// Device memory of the objects of a context (s is the context's stream).
+// dev_free waits for that stream only and keeps the block for a later
+// dev_alloc of the process (up to the block's size and at least half of it)
+// instead of returning it with hipFree, which waits for every stream of the
+// device: with many contexts running side by side, each scene's set-up and
+// teardown would otherwise stall all the others.  Kept blocks are returned to
+// HIP when an allocation runs out of memory.
+sphx_status dev_alloc(hipStream_t s, void **p, size_t bytes);
+void dev_free(hipStream_t s, void *p);
+
+// grow-only device scratch owned by a context (and used on its stream only)
 struct Scratch
 {
     void *ptr = nullptr;
     size_t bytes = 0;
+    hipStream_t stream = nullptr;
     sphx_status reserve(size_t n);
-    ~Scratch();
+    void release();
 };
 
 constexpr int kScalarSlots = 1024;
