@@ -1024,13 +1024,16 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
     }
     // rows out: each lane walks its own masks in candidate order, one
     // neighbour per slice row, so the rows are written whole
-    uint32_t P[R]; // this lane's flat-list offsets of its cell rows
+    // this lane's flat-list offsets of its cell rows, and for each row the
+    // shift from a flat slot to its particle_index position (mod 2^32)
+    uint32_t P[R], shift[R];
     {
         uint32_t acc = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             P[r] = acc;
+            shift[r] = s0[r] - acc;
             acc += len[r];
         }
     }
@@ -1052,15 +1055,12 @@ __global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
                     cur = my_masks[lane][++cur_q];
                 const uint32_t f = (cur_q << 6) + (uint32_t)(__ffsll((long long)cur) - 1);
                 cur &= cur - 1ull;
-                uint32_t row = 0, off = f;
+                // the last row starting at or before f (rows are in flat order)
+                uint32_t sh = shift[0];
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    if (f >= P[r])
-                    {
-                        row = (uint32_t)r;
-                        off = f - P[r];
-                    }
-                jj[u] = a.particle_index[pick<R>(s0, row) + off];
+                    sh = f >= P[r] ? shift[r] : sh;
+                jj[u] = a.particle_index[f + sh];
             }
         }
 #pragma unroll
