@@ -1128,7 +1128,7 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
     __syncthreads();
     const uint32_t count = valid ? s.neighbor_size[i] : 0u;
     const size_t row0 = valid ? (size_t)s.chunk_offset[i >> 6] + (i & 63u) : 0;
-    constexpr uint32_t kInBatch = 8; // row reads in flight per lane
+    constexpr uint32_t kInBatch = 16; // row reads in flight per lane
     constexpr uint32_t kLow = (1u << kSegShift) - 1u;
     // a list runs through its candidates in cell order, ascending index within
     // a cell: consecutive entries of one group are merged into one mask and
