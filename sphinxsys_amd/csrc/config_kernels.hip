@@ -715,8 +715,10 @@ __global__ void __launch_bounds__(kThreads) k_relation_build(BuildArgs b)
 // phase walks every lane's own masks in candidate order, one neighbour per
 // lane per slice row, so each slice row is written whole.
 //---------------------------------------------------------------------------
-constexpr uint32_t kTileWaves = 4;
-constexpr uint32_t kMaskSlots = 15;  // candidate batches of 64 per source: at most 960 candidates (30 KB of masks per block)
+// one wave per workgroup: a wave's LDS and slot are released as soon as it
+// ends, not with the slowest wave of a larger group (faster than 2, 4 or 8)
+constexpr uint32_t kTileWaves = 1;
+constexpr uint32_t kMaskSlots = 15;  // candidate batches of 64 per source: at most 960 candidates (7.5 KB of masks per wave)
 constexpr float kFarAway = 1.0e30f; // position of an unused candidate slot (squared distance overflows to inf)
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t col)
@@ -755,7 +757,7 @@ __device__ __forceinline__ uint32_t pick(const uint32_t (&v)[R], uint32_t idx)
 }
 
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_relation_tile(BuildArgs b)
+__global__ void __launch_bounds__(kTileWaves * 64) k_relation_tile(BuildArgs b)
 {
     constexpr int R = D == 3 ? 9 : 3; // cell rows of the search box, l -> m order
     constexpr int QMAX = 8;           // candidates of a cell kept in registers: 64 * QMAX
@@ -1488,7 +1490,7 @@ sphx_status relation_to_csr(sphx_relation *rel)
 template <int D>
 static sphx_status launch_tile(sphx_context *ctx, const BuildArgs &b, uint32_t nchunks)
 {
-    hipLaunchKernelGGL((k_relation_tile<D>), dim3(grid_for(nchunks, kTileWaves)), dim3(kThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((k_relation_tile<D>), dim3(grid_for(nchunks, kTileWaves)), dim3(kTileWaves * 64), 0, ctx->stream,
                        b);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
