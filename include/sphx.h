@@ -120,7 +120,11 @@ typedef struct sphx_body_desc {
 const char *sphx_backend_name(void);     /* "hip-gfx950" for the device library */
 const char *sphx_last_error(void);
 
-/* ---- context ------------------------------------------------------------ */
+/* ---- context ------------------------------------------------------------
+ * The *_destroy calls wait for their own context's stream only; the device
+ * memory and streams they release stay with the library for the next objects
+ * of the process (the memory goes back to HIP when an allocation of the
+ * library runs out of device memory). */
 sphx_status sphx_context_create(int device, sphx_context **out);
 sphx_status sphx_context_destroy(sphx_context *ctx);
 sphx_status sphx_context_synchronize(sphx_context *ctx);
