@@ -45,7 +45,18 @@ def _bytes(vec, scalar, rec):
 PASS1_BYTES = {"mixed": _bytes(12, 4, 16)[0], "fp64": _bytes(24, 8, 32)[0]}
 PASS2_BYTES = {"mixed": _bytes(12, 4, 16)[1], "fp64": _bytes(24, 8, 32)[1]}
 STEP_BYTES = {"mixed": 340.0, "fp64": 545.0}
-FLOP_PER_PAIR = 125.0
+# Algorithmic FP32 flops per neighbour pair (FMA = 2, 1/sqrt = 1), counted from
+# the pair arithmetic of the reference (acoustic_step_1st_half.hpp:89-111,157-180,
+# acoustic_step_2nd_half.hpp:53-73,117-137) with the tabulated kernel as a cubic:
+#  pass 1 inner: r (3), |r|^2 (5), 1/|r| (1), |r| and q (2), cubic (6), dW V (1),
+#    (p_i + p_j) dW V / |r| (3), force += s r (6), dC += (p_i - p_j) dW V (3)       = 30
+#  pass 1 wall: as inner plus e (3), the wall pressure p_w from the face acceleration (12)  = 45
+#  pass 2 inner: r, |r|^2, 1/|r|, |r|, q, cubic, dW V (19), e (3), v_i - v_j (3), u (5),
+#    div += u dW V (2), limiter PJump (4), s (1), force += s e (6)                  = 43
+#  pass 2 wall: as inner plus 2 (v_i - v_w) (3), e.n and its sign (7), n' (3), u on n' (5) = 57
+PASS1_FLOP_INNER, PASS1_FLOP_WALL = 30.0, 45.0
+PASS2_FLOP_INNER, PASS2_FLOP_WALL = 43.0, 57.0
+STEP_FLOP_PER_PAIR = PASS1_FLOP_INNER + PASS2_FLOP_INNER
 
 
 PROFILE = os.path.join(ROOT, "profiles", "dambreak3d_8M_mixed.md")
@@ -70,29 +81,28 @@ def profiled_traffic(stage):
     return None
 
 
-def run_gpu(lib, geom, precision, warmup, steps):
-    from sphinxsys_amd import _capi as capi
-    from sphinxsys_amd import cases
-    t0 = time.time()
-    sc = cases.DamBreak(lib, geom, precision)
-    sc.initialize()
-    log(f"  scene ready in {time.time() - t0:.1f}s ({sc.fluid.n} fluid, {sc.wall.n} wall)")
-    for _ in range(warmup):
-        sc.advection_step()
+def timed_window(sc, steps):
+    """K advection steps timed by the host clock (synchronised on both sides),
+    with HIP events around the two acoustic passes only (events around every
+    small kernel would slow the step by 2-3 %)."""
     sc.ctx.synchronize()
-    # HIP events around the two acoustic passes only during the timed steps
-    # (events around every small kernel would slow the step by 2-3 %)
     sc.ctx.profile(True, passes_only=True)
-    a0, i0 = sc.acoustic_steps, sc.iterations
+    a0, i0, t_phys = sc.acoustic_steps, sc.iterations, sc.time
     t0 = time.perf_counter()
     for _ in range(steps):
         sc.advection_step()
     sc.ctx.synchronize()
     wall = time.perf_counter() - t0
-    n_ac = sc.acoustic_steps - a0
-    n_adv = sc.iterations - i0
     passes = {k: sc.ctx.profile_read(k) for k in ("acoustic_pass1", "acoustic_pass2")}
-    # the whole stage breakdown over the next `steps` steps, outside the timed region
+    sc.ctx.profile(False)
+    return {"wall_s": wall, "acoustic_steps": sc.acoustic_steps - a0, "advection_steps": sc.iterations - i0,
+            "timed_passes": passes, "t_start": t_phys, "t_end": sc.time,
+            "sorts": sum(1 for k in range(i0, sc.iterations) if (k + 1) % 100 == 0)}
+
+
+def stage_breakdown(sc, steps):
+    """HIP events around every launch over `steps` advection steps (outside
+    any timed window): ms per stage."""
     sc.ctx.profile(True)
     for _ in range(steps):
         sc.advection_step()
@@ -102,30 +112,75 @@ def run_gpu(lib, geom, precision, warmup, steps):
         "relation_build", "relation_scan", "relation_stage", "cell_linked_list", "particle_sort",
         "time_step_reduce", "state_update")}
     sc.ctx.profile(False)
+    return prof
+
+
+def run_gpu(lib, geom, precision, warmup, steps, sustain_t=0.0):
+    """The scene on the GPU: a fresh-lattice window (after `warmup` steps) and,
+    with sustain_t > 0, a mid-run window after the run has passed physical time
+    sustain_t (started 40 advection steps after a ParticleSortCK, so that no
+    sort falls inside it); then one ParticleSortCK timed alone for the amortised
+    cost (the CK 3D case sorts every 100 advection steps, dambreak.cpp:213-216)."""
+    from sphinxsys_amd import cases
+    t0 = time.time()
+    sc = cases.DamBreak(lib, geom, precision)
+    sc.initialize()
+    log(f"  scene ready in {time.time() - t0:.1f}s ({sc.fluid.n} fluid, {sc.wall.n} wall)")
+    for _ in range(warmup):
+        sc.advection_step()
+    out = {"n_fluid": sc.fluid.n, "n_wall": sc.wall.n, "fresh": timed_window(sc, steps)}
+    window = out["fresh"]
+    if sustain_t > 0.0:
+        t0 = time.perf_counter()
+        last = t0
+        while sc.time < sustain_t or sc.iterations % 100 != 40:
+            sc.advection_step()
+            if time.perf_counter() - last > 10.0:
+                last = time.perf_counter()
+                log(f"  t = {sc.time:.4f} s after {sc.iterations} advection steps, {last - t0:.0f} s")
+        log(f"  mid-run window from t = {sc.time:.4f} s, advection step {sc.iterations}")
+        out["sustained"] = timed_window(sc, steps)
+        window = out["sustained"]
+    out["profile"] = stage_breakdown(sc, steps)
     inner_size, _, _ = sc.inner.read()
     wall_size, _, _ = sc.contact.read()
-    k_inner = float(inner_size.mean())
-    k_wall = float(wall_size.mean())
+    out["mean_neighbors_inner"] = float(inner_size.mean())
+    out["mean_neighbors_wall"] = float(wall_size.mean())
     probe, energy = sc.observe()
-    front = sc.front.exec()
-    out = {"wall_s": wall, "acoustic_steps": n_ac, "advection_steps": n_adv,
-           "n_fluid": sc.fluid.n, "n_wall": sc.wall.n, "profile": prof, "timed_passes": passes,
-           "mean_neighbors_inner": k_inner, "mean_neighbors_wall": k_wall,
-           "physical_time": sc.time, "probe_pressure": float(probe[0]), "energy": energy,
-           "front": front}
+    out["observables"] = {"physical_time": sc.time, "probe_pressure": float(probe[0]), "energy": energy,
+                          "front": sc.front.exec()}
+    # one ParticleSortCK alone, with events around its launches
+    sc.ctx.synchronize()
+    sc.ctx.profile(True)
+    sc.particle_sort.exec()
+    sc.ctx.synchronize()
+    out["sort_ms"] = sc.ctx.profile_read("particle_sort")[0]
+    sc.ctx.profile(False)
+    out["window"] = window
     sc.close()
     return out
 
 
-def run_cpu_baseline(geom, threads):
-    """The oracle (CPU restatement, FP64, std::thread pool) on the same 8M
-    workload: one full advection step after the initial configuration."""
-    from sphinxsys_amd import _capi as capi
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(geom, threads, precision):
+    """The oracle (the CPU restatement of the CK par_host path, std::thread pool
+    in place of TBB) on the same 8M workload: one full advection step after the
+    initial configuration, in FP64 (precision 0) or FP32 (precision 2)."""
     from sphinxsys_amd import cases
     from tests._util import oracle_lib
     lib = oracle_lib()
     lib.lib.sphx_oracle_set_threads(threads)
-    sc = cases.DamBreak(lib, geom, capi.FP64)
+    sc = cases.DamBreak(lib, geom, precision)
     sc.initialize()
     t0 = time.perf_counter()
     n = sc.advection_step()
@@ -221,7 +276,10 @@ def main():
     ap.add_argument("--dp", type=float, default=0.00448, help="particle spacing")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="threads of the CPU baseline (default: every CPU this process may run on)")
+    ap.add_argument("--sustain", type=float, default=0.6, metavar="T",
+                    help="physical time after which the mid-run (headline) window is timed; 0: fresh only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--full-run", type=float, default=0.0, metavar="T_END",
@@ -317,30 +375,48 @@ def main():
         return
 
     log("FP32-mixed run")
-    mixed = run_gpu(lib, geom, capi.MIXED, args.warmup, args.steps)
+    mixed = run_gpu(lib, geom, capi.MIXED, args.warmup, args.steps, sustain_t=args.sustain)
     fp64 = None
     if not args.no_fp64:
         log("FP64 run")
         fp64 = run_gpu(lib, geom, capi.FP64, args.warmup, args.steps)
-    cpu = None
+    cpu = {}
     if not args.no_cpu:
-        log(f"CPU baseline (oracle, {args.cpu_threads} threads)")
-        cpu = run_cpu_baseline(geom, args.cpu_threads)
+        for name, prec in (("fp64", capi.FP64), ("fp32", capi.FP32)):
+            log(f"CPU baseline (oracle, {name}, {args.cpu_threads} threads)")
+            cpu[name] = run_cpu_baseline(geom, args.cpu_threads, prec)
 
     n_f = mixed["n_fluid"]
-    value = n_f * mixed["acoustic_steps"] / mixed["wall_s"]
-    ms_per_step = 1000.0 * mixed["wall_s"] / args.steps
 
-    # dominant kernel: the acoustic pass with the larger total time in the timed steps
-    p1_ms, p1_n = mixed["timed_passes"]["acoustic_pass1"]
-    p2_ms, p2_n = mixed["timed_passes"]["acoustic_pass2"]
+    def rate(run, window):
+        """particle-steps/s of a timed window with the amortised ParticleSortCK
+        (one sort per 100 advection steps, as the CK 3D case) added to its wall
+        time; sorts inside the window are already in it."""
+        w = run[window]
+        extra = (w["advection_steps"] / 100.0 - w["sorts"]) * run["sort_ms"] * 1e-3
+        wall = w["wall_s"] + max(extra, 0.0)
+        return n_f * w["acoustic_steps"] / wall, wall
+
+    head = "sustained" if "sustained" in mixed else "fresh"
+    value, wall = rate(mixed, head)
+    ms_per_step = 1000.0 * wall / args.steps
+    win = mixed[head]
+
+    # dominant kernel: the acoustic pass with the larger total time in the headline window
+    p1_ms, p1_n = win["timed_passes"]["acoustic_pass1"]
+    p2_ms, p2_n = win["timed_passes"]["acoustic_pass2"]
+    k_in, k_w = mixed["mean_neighbors_inner"], mixed["mean_neighbors_wall"]
     if p2_ms >= p1_ms:
         dom, dom_ms, dom_n, dom_b = "acoustic_pass2", p2_ms, p2_n, PASS2_BYTES["mixed"]
+        dom_f = PASS2_FLOP_INNER * k_in + PASS2_FLOP_WALL * k_w
     else:
         dom, dom_ms, dom_n, dom_b = "acoustic_pass1", p1_ms, p1_n, PASS1_BYTES["mixed"]
+        dom_f = PASS1_FLOP_INNER * k_in + PASS1_FLOP_WALL * k_w
     avg_ms = dom_ms / max(dom_n, 1)
     achieved = dom_b * n_f / (avg_ms * 1e-3) / 1e9
-    k_bar = mixed["mean_neighbors_inner"] + mixed["mean_neighbors_wall"]
+    flops = dom_f * n_f / (avg_ms * 1e-3) / 1e12
+    hbm_frac, valu_frac = achieved / HBM_PEAK_GBS, flops * 1e12 / FP32_VALU_PEAK
+    k_bar = k_in + k_w
     line = {
         "metric": METRIC,
         "value": value,
@@ -359,48 +435,65 @@ def main():
                         f"{n_f} fluid + {mixed['n_wall']} wall particles, FP32-mixed",
             "fluid_particles": n_f,
             "wall_particles": mixed["n_wall"],
-            "acoustic_steps_timed": mixed["acoustic_steps"],
+            "acoustic_steps_timed": win["acoustic_steps"],
+            "window": f"{head}: advection steps {args.warmup if head == 'fresh' else 'from ' + str(round(win['t_start'], 4)) + ' s'}"
+                      f" to t = {win['t_end']:.4f} s, ParticleSortCK amortised at 1 per 100 advection steps",
             "mean_neighbors": k_bar,
             "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": "valu" if valu_frac > hbm_frac else "hbm",
             "kernel": dom,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": hbm_frac,
             "traffic": profiled_traffic(dom),
             "traffic_source": "profiles/dambreak3d_8M_mixed.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)",
             "bytes_per_particle": dom_b,
             "avg_launch_ms": avg_ms,
+            "valu": {"achieved": flops, "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": valu_frac,
+                     "flop_per_particle": dom_f,
+                     "note": "algorithmic FP32 flops of the pair loops (FMA = 2) at the measured mean "
+                             "neighbour counts; the published peak counts packed FP32, which this build "
+                             "does not issue, so the issue-limited ceiling is half of it"},
         },
         "step_roofline": {
             "hbm_frac": STEP_BYTES["mixed"] * value / (HBM_PEAK_GBS * 1e9),
-            "valu_frac": FLOP_PER_PAIR * k_bar * value / FP32_VALU_PEAK,
+            "valu_frac": STEP_FLOP_PER_PAIR * k_bar * value / FP32_VALU_PEAK,
             "bytes_per_particle_step": STEP_BYTES["mixed"],
-            "flop_per_particle_step": FLOP_PER_PAIR * k_bar,
+            "flop_per_particle_step": STEP_FLOP_PER_PAIR * k_bar,
         },
+        "windows": {name: {"value": rate(mixed, name)[0], "ms_per_step": 1000.0 * rate(mixed, name)[1] / args.steps,
+                           "t_start": mixed[name]["t_start"], "t_end": mixed[name]["t_end"],
+                           "acoustic_steps": mixed[name]["acoustic_steps"]}
+                    for name in ("fresh", "sustained") if name in mixed},
+        "sort_ms": mixed["sort_ms"],
         "stage_ms": {k: round(v[0], 3) for k, v in mixed["profile"].items()},
         "stage_ms_source": f"HIP events around every launch over the {args.steps} advection steps after the "
-                           f"timed ones (the timed steps time only the two acoustic passes)",
-        "observables": {"physical_time": mixed["physical_time"],
-                        "probe_pressure": mixed["probe_pressure"], "front": mixed["front"]},
+                           f"headline window (the timed steps time only the two acoustic passes)",
+        "observables": mixed["observables"],
     }
     if fp64:
-        v64 = n_f * fp64["acoustic_steps"] / fp64["wall_s"]
-        line["fp64"] = {"value": v64, "ms_per_step": 1000.0 * fp64["wall_s"] / args.steps,
-                        "acoustic_steps_timed": fp64["acoustic_steps"],
+        v64 = rate(fp64, "fresh")[0]
+        line["fp64"] = {"value": v64, "window": "fresh",
+                        "ms_per_step": 1000.0 * rate(fp64, "fresh")[1] / args.steps,
+                        "acoustic_steps_timed": fp64["fresh"]["acoustic_steps"], "sort_ms": fp64["sort_ms"],
                         "stage_ms": {k: round(v[0], 3) for k, v in fp64["profile"].items()},
-                        "probe_pressure": fp64["probe_pressure"], "front": fp64["front"]}
-        line["mixed_over_fp64"] = value / v64
+                        "observables": fp64["observables"]}
+        # like for like: both fresh-lattice windows
+        line["mixed_over_fp64"] = rate(mixed, "fresh")[0] / v64
     if cpu:
+        c64 = cpu["fp64"]
         line["cpu_baseline"] = {
-            "value": cpu["value"], "unit": "particle-steps/s", "cores": args.cpu_threads,
+            "value": c64["value"], "unit": "particle-steps/s", "cores": args.cpu_threads,
             "kind": "port",
-            "sample": f"one advection step ({cpu['acoustic_steps']} acoustic sub-steps + "
-                      f"configuration update) of the same 8M workload, FP64, "
-                      f"{cpu['wall_s']:.1f}s"}
+            "sample": f"one advection step ({c64['acoustic_steps']} acoustic sub-steps + configuration update) "
+                      f"of the same 8M workload from the initial lattice, FP64, {c64['wall_s']:.1f}s on "
+                      f"{args.cpu_threads} threads of {cpu_model()} (nproc {os.cpu_count()})",
+            "fp32": {"value": cpu["fp32"]["value"], "wall_s": cpu["fp32"]["wall_s"],
+                     "acoustic_steps": cpu["fp32"]["acoustic_steps"]},
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
     print(json.dumps(line), flush=True)
 
 

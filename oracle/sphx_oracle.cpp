@@ -1215,6 +1215,17 @@ sphx_status sphx_free_surface_indication(sphx_relation *inner, sphx_relation *co
     });
 }
 
+// Not in the reference (its max reductions drop NaN): a time step that is not
+// finite and positive comes from a particle with a non-finite velocity, force
+// or compression, and is reported instead of returned
+static sphx_status finite_step(double dt, const char *which)
+{
+    if (std::isfinite(dt) && dt > 0.0)
+        return SPHX_OK;
+    return fail(SPHX_E_NONFINITE, std::string("non-finite ") + which +
+                                      " time step: a particle's velocity, force or compression is not finite");
+}
+
 // AcousticTimeStepCK, fluid_time_step_ck.hpp:30-57 (h_min = reference smoothing length)
 sphx_status sphx_acoustic_time_step(sphx_body *body, double cfl, double *dt)
 {
@@ -1226,10 +1237,14 @@ sphx_status sphx_acoustic_time_step(sphx_body *body, double cfl, double *dt)
         R m = block_max<R>(B.n, [&](size_t i) {
             R force_norm = norm(B.force[i] + B.force_prior[i]);
             R acc_scale = std::sqrt(4.0 * h_min * force_norm / B.mass[i]);
-            return std::max(c0 + norm(B.vel[i]), acc_scale);
+            R signal = c0 + norm(B.vel[i]);
+            // a particle with a non-finite state takes +inf (the step becomes zero)
+            if (!std::isfinite(signal) || !std::isfinite(acc_scale) || !std::isfinite(B.C[i]))
+                return std::numeric_limits<R>::infinity();
+            return std::max(signal, acc_scale);
         });
         *dt = (double)(R(cfl) * h_min / (m + TinyReal<R>()));
-        return SPHX_OK;
+        return finite_step(*dt, "acoustic");
     });
 }
 // AdvectionTimeStepCK, fluid_time_step_ck.cpp:23-26, fluid_time_step_ck.h:107-110
@@ -1239,9 +1254,12 @@ sphx_status sphx_advection_time_step(sphx_body *body, double u_ref, double cfl, 
         return fail(SPHX_E_ARG, "null dt");
     return with_body(body, [&](auto &B) {
         using R = typename std::decay<decltype(B)>::type::Real;
-        R m = block_max<R>(B.n, [&](size_t i) { return squaredNorm(B.vel[i]); });
+        R m = block_max<R>(B.n, [&](size_t i) {
+            R v2 = squaredNorm(B.vel[i]);
+            return std::isfinite(v2) ? v2 : std::numeric_limits<R>::infinity();
+        });
         *dt = (double)(R(cfl) * B.h / (std::max(R(std::sqrt(m)), R(u_ref)) + TinyReal<R>()));
-        return SPHX_OK;
+        return finite_step(*dt, "advection");
     });
 }
 // TotalMechanicalEnergyCK, general_reduce_ck.h:86-90; Gravity::getPotential external_force.h:53-56

@@ -112,6 +112,18 @@ __device__ __forceinline__ bool step_dt(const FluidArgs<T> &a, double &dt_d)
     return true;
 }
 
+// per-particle signal speed max(c0 + |v|, sqrt(4 h |F + F_prior| / m)) of
+// AcousticTimeStepCK for the max reduction, or +inf when it or the particle's
+// compression is not finite: a diverged particle then makes the acoustic time
+// step zero (reported as SPHX_E_NONFINITE) instead of dropping out of the max
+template <class T>
+__device__ __forceinline__ double signal_or_inf(T c_plus_v, T acc_scale, T compression)
+{
+    if (!isfinite(c_plus_v) || !isfinite(acc_scale) || !isfinite(compression))
+        return __builtin_huge_val();
+    return (double)dmax(c_plus_v, acc_scale);
+}
+
 // position/volume record of particle j
 template <class T, int D>
 struct PV
@@ -696,7 +708,7 @@ k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_s
             T fn = sqrt(D == 3 ? gx * gx + gy * gy + gz * gz : gx * gx + gy * gy);
             T acc_scale = sqrt(T(4) * a.h * fn / m_i);
             T vn = sqrt(D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y);
-            signal = (double)dmax(a.c0 + vn, acc_scale);
+            signal = signal_or_inf(a.c0 + vn, acc_scale, c);
         }
     }
     if (FUSE_CFL)
@@ -1410,7 +1422,7 @@ __global__ void __launch_bounds__(kThreads) k_signal_max(FluidArgs<typename Prec
         T acc_scale = sqrt(T(4) * a.h * fn / m_i);
         Rec4<T> vi = a.vel[i];
         T vn = sqrt(D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y);
-        v = (double)dmax(a.c0 + vn, acc_scale);
+        v = signal_or_inf(a.c0 + vn, acc_scale, T(0));
     }
     block_max_to(v, slot);
 }
@@ -1425,7 +1437,8 @@ __global__ void __launch_bounds__(kThreads) k_speed_max(FluidArgs<typename Prec<
     if (i < a.n)
     {
         Rec4<T> vi = a.vel[i];
-        v = (double)(D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y);
+        const T v2 = D == 3 ? vi.x * vi.x + vi.y * vi.y + vi.z * vi.z : vi.x * vi.x + vi.y * vi.y;
+        v = isfinite(v2) ? (double)v2 : __builtin_huge_val();
     }
     block_max_to(v, slot);
 }
@@ -1558,13 +1571,15 @@ __global__ void k_step_begin(unsigned long long *scalars, double u_ref, double c
     unsigned long long *st = scalars + SLOT_STEP;
     const double m = lanes_max_reset(scalars + SLOT_MAX);
     // AdvectionTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.cpp:23-26
-    put_double(st + STEP_ADV_DT, cfl * h / (fmax(sqrt(m), u_ref) + 2.71051e-20));
+    const double adv_dt = cfl * h / (fmax(sqrt(m), u_ref) + 2.71051e-20);
+    put_double(st + STEP_ADV_DT, adv_dt);
     put_double(st + STEP_DT, 0.0);
     put_double(st + STEP_RELAXATION, 0.0);
     put_double(st + STEP_TIME, time);
     st[STEP_COUNT] = 0;
     st[STEP_ACTIVE] = 0;
-    st[STEP_ERROR] = 0;
+    // a non-finite velocity made the speed max +inf and the step zero
+    st[STEP_ERROR] = isfinite(adv_dt) && adv_dt > 0.0 ? 0ull : 1ull;
 }
 
 __global__ void k_substep_begin(unsigned long long *scalars, double cfl, double h)

@@ -1272,6 +1272,17 @@ static sphx_status read_max(sphx_context *ctx, double *out)
     return SPHX_OK;
 }
 
+// the max reductions of the time steps take +inf for a particle with a
+// non-finite velocity, force or compression, which makes the step zero
+static sphx_status finite_step(double dt, const char *which)
+{
+    if (std::isfinite(dt) && dt > 0.0)
+        return SPHX_OK;
+    set_error(std::string("non-finite ") + which + " time step: a particle's velocity, force or compression "
+              "is not finite");
+    return SPHX_E_NONFINITE;
+}
+
 sphx_status sphx_acoustic_time_step(sphx_body *b, double cfl, double *dt)
 {
     if (!b || !dt)
@@ -1287,7 +1298,7 @@ sphx_status sphx_acoustic_time_step(sphx_body *b, double cfl, double *dt)
         return s;
     // AcousticTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.hpp:30-36
     *dt = cfl * b->h / (m + 2.71051e-20);
-    return SPHX_OK;
+    return finite_step(*dt, "acoustic");
 }
 
 sphx_status sphx_advection_time_step(sphx_body *b, double u_ref, double cfl, double *dt)
@@ -1305,7 +1316,7 @@ sphx_status sphx_advection_time_step(sphx_body *b, double u_ref, double cfl, dou
         return s;
     // AdvectionTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.cpp:23-26
     *dt = cfl * b->h / (std::max(std::sqrt(m), u_ref) + 2.71051e-20);
-    return SPHX_OK;
+    return finite_step(*dt, "advection");
 }
 
 sphx_status sphx_total_mechanical_energy(sphx_body *b, const double *g, double *e)

@@ -19,6 +19,9 @@ def oracle_lib():
     lib = SphxLibrary(ORACLE_SO)
     assert lib.backend == "oracle-cpu"
     lib.lib.sphx_oracle_set_threads.restype = int
+    if not os.environ.get("SPHX_ORACLE_THREADS"):
+        # the CPUs this process may run on (a GPU host's share of its cores), at most 16
+        lib.lib.sphx_oracle_set_threads(min(16, len(os.sched_getaffinity(0))))
     return lib
 
 

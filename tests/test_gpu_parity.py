@@ -288,6 +288,24 @@ def test_dambreak_2d_short_run(hip, oracle, precision):
     pg, eg = gpu.observe()
     pc, ec = cpu.observe()
     assert eg == pytest.approx(ec, rel=1e-10 if fp64 else 1e-3)
+    # the wall probe (no water there yet at t ~ 0.2: both zero) ...
+    assert np.allclose(pg, pc, rtol=1e-9 if fp64 else 2e-2, atol=1e-12)
+    # ... and Interpolation<Contact<Real>> at points inside the water column, on
+    # the same states: pressure, density and velocity
+    pts = np.array([[0.3, 0.2], [1.0, 0.5], [1.7, 0.1]])
+    probes = {}
+    for key, sc in (("gpu", gpu), ("cpu", cpu)):
+        obs = ck.Body(sc.ctx, 2, len(pts), geom.dp, capi.BODY_OBSERVER, sc.precision, rho0=geom.rho0, c0=geom.c0)
+        obs.write("Position", pts)
+        obs.write("VolumetricMeasure", np.full(len(pts), geom.dp ** 2))
+        rel = ck.Contact(obs, sc.fluid, sc.fluid_cll)
+        ck.UpdateRelation(rel).exec()
+        probes[key] = [ck.InterpolationContact(rel, f).exec() for f in ("Pressure", "Density", "Velocity")]
+        rel.close()
+        obs.close()
+    assert np.all(np.abs(probes["cpu"][0]) > 1e-3)  # the points see water
+    for g, c, tol in zip(probes["gpu"], probes["cpu"], ((1e-9, 2e-2), (1e-12, 1e-5), (1e-9, 3e-2))):
+        assert rel_l2(g, c) < tol[0 if fp64 else 1], (rel_l2(g, c), g, c)
     gpu.close()
     cpu.close()
 
