@@ -159,6 +159,15 @@ sphx_status sphx_cell_linked_list_mesh(const sphx_cell_linked_list *cll, int *al
 /* reference: UpdateCellLinkedList<P, RealBody>::exec,
  * shared_ck/particle_dynamics/configuration_dynamics/update_cell_linked_list.hpp:75-106 */
 sphx_status sphx_update_cell_linked_list(sphx_cell_linked_list *cll);
+/* No reference counterpart: sphx_update_cell_linked_list for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result; its workspace is a fixed device array of the
+ * library (about 32 MiB per device, held until the process ends) instead of
+ * the context's grow-only scratch, and it records no profiling events.  At
+ * most 1,048,576 particles and 4,194,304 cells (SPHX_E_CAPACITY beyond).
+ * Every call on one device shares that workspace: calls that overlap on one
+ * device (contexts driven from several host threads) race. */
+sphx_status sphx_update_cell_linked_list_noalloc(sphx_cell_linked_list *cll);
 /* cell_offset[n_cells+1], particle_index[n_particles] (per-cell index order
  * is ascending particle index; the reference fills it with relaxed atomics). */
 sphx_status sphx_cell_linked_list_read(sphx_cell_linked_list *cll, uint32_t *cell_offset,

@@ -927,6 +927,8 @@ sphx_status sphx_update_cell_linked_list(sphx_cell_linked_list *c)
         return SPHX_OK;
     });
 }
+// the CPU restatement holds no device workspace: the same update
+sphx_status sphx_update_cell_linked_list_noalloc(sphx_cell_linked_list *c) { return sphx_update_cell_linked_list(c); }
 sphx_status sphx_cell_linked_list_read(sphx_cell_linked_list *c, uint32_t *cell_offset, size_t n_off,
                                        uint32_t *particle_index, size_t n_idx)
 {

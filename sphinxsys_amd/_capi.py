@@ -36,7 +36,7 @@ SYMBOLS = [
     "sphx_context_synchronize", "sphx_body_create", "sphx_body_destroy", "sphx_body_write",
     "sphx_body_read", "sphx_body_smoothing_length", "sphx_cell_linked_list_create",
     "sphx_cell_linked_list_destroy", "sphx_cell_linked_list_mesh",
-    "sphx_update_cell_linked_list", "sphx_cell_linked_list_read",
+    "sphx_update_cell_linked_list", "sphx_update_cell_linked_list_noalloc", "sphx_cell_linked_list_read",
     "sphx_relation_create_inner", "sphx_relation_create_contact", "sphx_relation_destroy",
     "sphx_update_relation", "sphx_relation_read", "sphx_particle_sort", "sphx_gravity_force",
     "sphx_density_regularization", "sphx_advection_step_setup",
@@ -96,6 +96,7 @@ class SphxLibrary:
             "sphx_cell_linked_list_destroy": (C.c_int, [_vp]),
             "sphx_cell_linked_list_mesh": (C.c_int, [_vp, C.POINTER(C.c_int), _dp, _dp]),
             "sphx_update_cell_linked_list": (C.c_int, [_vp]),
+            "sphx_update_cell_linked_list_noalloc": (C.c_int, [_vp]),
             "sphx_cell_linked_list_read": (C.c_int, [_vp, _up, C.c_size_t, _up, C.c_size_t]),
             "sphx_relation_create_inner": (C.c_int, [_vp, _vp, _pp]),
             "sphx_relation_create_contact": (C.c_int, [_vp, _vp, _vp, _pp]),

@@ -9,6 +9,8 @@
 
 #include "sphx_internal.hpp"
 
+#include <mutex>
+
 namespace sphx
 {
 namespace
@@ -1275,7 +1277,7 @@ sphx_status fill_iota(sphx_context *ctx, uint32_t *a, size_t n)
 }
 
 sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *out, size_t n,
-                               uint32_t *total_host)
+                               uint32_t *total_host, uint32_t *sums)
 {
     if (n == 0)
     {
@@ -1284,10 +1286,13 @@ sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *
         return SPHX_OK;
     }
     uint32_t ntiles = grid_for(n, kTile);
-    sphx_status s = ctx->scan_blocks.reserve((size_t)ntiles * sizeof(uint32_t));
-    if (s)
-        return s;
-    uint32_t *sums = (uint32_t *)ctx->scan_blocks.ptr;
+    sphx_status s;
+    if (!sums)
+    {
+        if ((s = ctx->scan_blocks.reserve((size_t)ntiles * sizeof(uint32_t))))
+            return s;
+        sums = (uint32_t *)ctx->scan_blocks.ptr;
+    }
     unsigned long long *tot = ctx->scalars_d + SLOT_TOTAL;
     if (ntiles == 1)
         hipLaunchKernelGGL(k_scan_one_tile, dim3(1), dim3(kThreads), 0, ctx->stream, in, out, n, tot);
@@ -1307,26 +1312,38 @@ sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *
     return SPHX_OK;
 }
 
-sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits)
+sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits,
+                             const CllWork *work)
 {
     if (n <= 1 || bits <= 0)
         return SPHX_OK;
     sphx_status s;
-    if ((s = ctx->sort_keys.reserve(n * sizeof(uint32_t))) || (s = ctx->sort_vals.reserve(n * sizeof(uint32_t))))
-        return s;
     uint32_t ntiles = grid_for(n, kTile);
     size_t nhist = (size_t)kBuckets * ntiles;
-    if ((s = ctx->sort_hist.reserve(nhist * sizeof(uint32_t))))
-        return s;
-    uint32_t *hist = (uint32_t *)ctx->sort_hist.ptr;
+    uint32_t *hist, *k1, *v1, *sums = nullptr;
+    if (work)
+    {
+        hist = work->hist;
+        k1 = work->sort_keys;
+        v1 = work->sort_vals;
+        sums = work->scan_sums;
+    }
+    else
+    {
+        if ((s = ctx->sort_keys.reserve(n * sizeof(uint32_t))) || (s = ctx->sort_vals.reserve(n * sizeof(uint32_t))) ||
+            (s = ctx->sort_hist.reserve(nhist * sizeof(uint32_t))))
+            return s;
+        hist = (uint32_t *)ctx->sort_hist.ptr;
+        k1 = (uint32_t *)ctx->sort_keys.ptr;
+        v1 = (uint32_t *)ctx->sort_vals.ptr;
+    }
     uint32_t *k0 = keys, *v0 = vals;
-    uint32_t *k1 = (uint32_t *)ctx->sort_keys.ptr, *v1 = (uint32_t *)ctx->sort_vals.ptr;
     int passes = (bits + kRadixBits - 1) / kRadixBits;
     for (int p = 0; p < passes; ++p)
     {
         int shift = p * kRadixBits;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kThreads), 0, ctx->stream, k0, n, shift, hist, ntiles);
-        if ((s = exclusive_scan_u32(ctx, hist, hist, nhist, nullptr)))
+        if ((s = exclusive_scan_u32(ctx, hist, hist, nhist, nullptr, sums)))
             return s;
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kThreads), 0, ctx->stream, k0, v0, k1, v1, n, shift,
                            hist, ntiles);
@@ -1355,11 +1372,13 @@ static int bits_for(uint64_t max_value)
 // UpdateCellLinkedList<P,RealBody>::exec (update_cell_linked_list.hpp:75-106):
 // per-cell counts + exclusive scan give CellOffset; a stable sort of the cell
 // keys gives ParticleIndex in ascending particle order inside each cell.
-sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
+sphx_status cell_linked_list_update(sphx_cell_linked_list *cll, const CllWork *work)
 {
     sphx_body *b = cll->body;
     sphx_context *ctx = b->ctx;
-    ProfileScope prof(ctx, "cell_linked_list");
+    // the allocation-free form records no profiling events (their creation allocates)
+    ProfileScope prof(ctx, work ? nullptr : "cell_linked_list");
+    uint32_t *sums = work ? work->scan_sums : nullptr;
     const size_t nc = cll->ncells;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nc + 1, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,
                        nc + 1, 0u);
@@ -1375,22 +1394,26 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
                                pos_ptrs(b), n, cll->mesh, cll->keys, cll->cell_offset);
         SPHX_HIP(hipGetLastError());
     }
-    sphx_status s = exclusive_scan_u32(ctx, cll->cell_offset, cll->cell_offset, nc + 1, nullptr);
+    sphx_status s = exclusive_scan_u32(ctx, cll->cell_offset, cll->cell_offset, nc + 1, nullptr, sums);
     if (s || n == 0)
         return s;
     // particles by cell, ascending index within a cell: the runs of equal
     // cells in particle order (as many as cells, plus two per particle that
     // left its run since the last sort) are sorted stably by cell, and each
     // particle is placed at its run's output offset plus its place in the run
-    if ((s = ctx->cll_runs.reserve((size_t)6 * n * sizeof(uint32_t))))
-        return s;
-    uint32_t *ex = (uint32_t *)ctx->cll_runs.ptr;
+    uint32_t *ex = work ? work->runs : nullptr;
+    if (!ex)
+    {
+        if ((s = ctx->cll_runs.reserve((size_t)6 * n * sizeof(uint32_t))))
+            return s;
+        ex = (uint32_t *)ctx->cll_runs.ptr;
+    }
     uint32_t *run_key = ex + n, *run_idx = run_key + n, *run_start = run_idx + n, *len = run_start + n,
              *run_off = len + n;
     const dim3 g(grid_for(n, 256)), blk(256);
     hipLaunchKernelGGL(k_run_flags, g, blk, 0, ctx->stream, cll->keys, n, ex);
     uint32_t nruns = 0;
-    if ((s = exclusive_scan_u32(ctx, ex, ex, n, &nruns)))
+    if ((s = exclusive_scan_u32(ctx, ex, ex, n, &nruns, sums)))
         return s;
     hipLaunchKernelGGL(k_run_emit, g, blk, 0, ctx->stream, cll->keys, ex, n, run_key, run_idx, run_start);
     SPHX_HIP(hipGetLastError());
@@ -1402,11 +1425,11 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
     }
     else
     {
-        if ((s = radix_sort_pairs(ctx, run_key, run_idx, nruns, cll->key_bits)))
+        if ((s = radix_sort_pairs(ctx, run_key, run_idx, nruns, cll->key_bits, work)))
             return s;
         const dim3 gr(grid_for(nruns, 256));
         hipLaunchKernelGGL(k_run_len, gr, blk, 0, ctx->stream, run_idx, run_start, nruns, n, len);
-        if ((s = exclusive_scan_u32(ctx, len, len, nruns, nullptr)))
+        if ((s = exclusive_scan_u32(ctx, len, len, nruns, nullptr, sums)))
             return s;
         hipLaunchKernelGGL(k_run_off, gr, blk, 0, ctx->stream, run_idx, len, nruns, run_off);
     }
@@ -1416,6 +1439,78 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll)
     cll->built = true;
     ++cll->build_count;
     return SPHX_OK;
+}
+
+//---------------------------------------------------------------------------
+// Allocation-free cell-list update: the workspace of the run placement, the
+// radix passes and the scans lives in one file-scope device array of fixed
+// size.  Each device holds its own copy from module load to process end; its
+// device address is looked up once per device (noalloc_scratch_init, called
+// by the library's context creation with the device current) and only that
+// address is handed to kernels and copies.  Calls on one device share the
+// array, so calls that overlap on one device race.
+//---------------------------------------------------------------------------
+namespace
+{
+constexpr size_t kNoAllocScanSums = (kNoAllocMaxCells + 1 + kTile - 1) / kTile; // also >= tiles of n
+constexpr size_t kNoAllocHist = (size_t)kBuckets * ((kNoAllocMaxParticles + kTile - 1) / kTile);
+constexpr size_t kNoAllocWords = 8ull * kNoAllocMaxParticles + kNoAllocScanSums + kNoAllocHist;
+static_assert(kNoAllocScanSums >= (kNoAllocMaxParticles + kTile - 1) / kTile, "scan sums cover n");
+static_assert((kNoAllocHist + kTile - 1) / kTile <= kNoAllocScanSums, "scan sums cover the histogram scan");
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+uint32_t *g_noalloc_ptr[kMaxDevices] = {};
+} // namespace
+
+// 8 n + scan sums + histogram words for n <= kNoAllocMaxParticles (about 32 MiB)
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+sphx_status noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+    {
+        set_error("device index beyond the allocation-free workspace table");
+        return SPHX_E_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[device])
+        return SPHX_OK;
+    void *p = nullptr;
+    SPHX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[device] = (uint32_t *)p;
+    return SPHX_OK;
+}
+
+sphx_status cell_linked_list_update_noalloc(sphx_cell_linked_list *cll)
+{
+    const int device = cll->body->ctx->device;
+    uint32_t *base = nullptr;
+    if (device >= 0 && device < kMaxDevices)
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        base = g_noalloc_ptr[device];
+    }
+    if (!base)
+    {
+        set_error("allocation-free workspace not set up on this device (context creation does it)");
+        return SPHX_E_STATE;
+    }
+    const size_t n = cll->body->n;
+    if (n > kNoAllocMaxParticles || cll->ncells > kNoAllocMaxCells)
+    {
+        set_error("allocation-free cell-list update: " + std::to_string(n) + " particles and " +
+                  std::to_string(cll->ncells) + " cells exceed its workspace (at most " +
+                  std::to_string(kNoAllocMaxParticles) + " particles and " + std::to_string(kNoAllocMaxCells) +
+                  " cells); use sphx_update_cell_linked_list");
+        return SPHX_E_CAPACITY;
+    }
+    CllWork w;
+    w.runs = base;                              // 6 n
+    w.sort_keys = base + 6ull * kNoAllocMaxParticles; // n
+    w.sort_vals = base + 7ull * kNoAllocMaxParticles; // n
+    w.scan_sums = base + 8ull * kNoAllocMaxParticles;
+    w.hist = w.scan_sums + kNoAllocScanSums;
+    return cell_linked_list_update(cll, &w);
 }
 
 // near flags of a (static) target cell linked list, recomputed after each rebuild

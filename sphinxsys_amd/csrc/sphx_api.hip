@@ -167,7 +167,8 @@ static hipEvent_t pooled_event(sphx_context *ctx)
 
 ProfileScope::ProfileScope(sphx_context *c, const char *s) : ctx(c), stage(s)
 {
-    const bool timed = ctx->profiling && (!ctx->profile_passes_only || std::strncmp(s, "acoustic_pass", 13) == 0);
+    // a null stage records nothing
+    const bool timed = s && ctx->profiling && (!ctx->profile_passes_only || std::strncmp(s, "acoustic_pass", 13) == 0);
     start = timed ? pooled_event(ctx) : nullptr;
     if (start)
         (void)hipEventRecord(start, ctx->stream);
@@ -459,12 +460,30 @@ sphx_status sphx_context_create(int device, sphx_context **out)
     if (device < 0 || device >= count)
         return arg_error("device index out of range");
     SPHX_HIP(hipSetDevice(device));
+    sphx_status st;
+    // the workspace address of the allocation-free entry points, once per device
+    if ((st = noalloc_scratch_init(device)))
+        return st;
     std::unique_ptr<sphx_context> ctx(new sphx_context());
     ctx->device = device;
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         ctx->max_lds = (size_t)lds;
-    sphx_status st;
+    // on a failure below, the stream and the pinned mirror go back to the pools
+    struct Release
+    {
+        sphx_context *c;
+        ~Release()
+        {
+            if (!c)
+                return;
+            std::lock_guard<std::mutex> lock(g_reuse_mutex);
+            if (c->stream)
+                g_free_streams[c->device].push_back(c->stream);
+            if (c->scalars_h)
+                g_free_mirrors.push_back(c->scalars_h);
+        }
+    } release{ctx.get()};
     {
         std::lock_guard<std::mutex> lock(g_reuse_mutex);
         auto &streams = g_free_streams[device];
@@ -492,6 +511,7 @@ sphx_status sphx_context_create(int device, sphx_context **out)
     SPHX_HIP(hipMemcpyAsync(ctx->scalars_d, ctx->scalars_h, kScalarSlots * sizeof(unsigned long long),
                             hipMemcpyHostToDevice, ctx->stream));
     SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    release.c = nullptr;
     *out = ctx.release();
     return SPHX_OK;
 }
@@ -829,6 +849,18 @@ sphx_status sphx_update_cell_linked_list(sphx_cell_linked_list *c)
         return arg_error("null cell linked list");
     SPHX_HIP(hipSetDevice(c->body->ctx->device));
     sphx_status s = cell_linked_list_update(c);
+    if (s)
+        return s;
+    c->built_version = c->body->config_version;
+    return SPHX_OK;
+}
+
+sphx_status sphx_update_cell_linked_list_noalloc(sphx_cell_linked_list *c)
+{
+    if (!c)
+        return arg_error("null cell linked list");
+    SPHX_HIP(hipSetDevice(c->body->ctx->device));
+    sphx_status s = cell_linked_list_update_noalloc(c);
     if (s)
         return s;
     c->built_version = c->body->config_version;

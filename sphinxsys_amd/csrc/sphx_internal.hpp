@@ -194,10 +194,29 @@ struct sphx_relation
 namespace sphx
 {
 // ---- configuration kernels (config_kernels.hip) ----
+// Caller-provided device workspace of the cell-list update (null members: the
+// context's grow-only scratch is used and grown with dev_alloc).  The caller
+// guarantees the sizes: runs 6 n, sort_keys and sort_vals n, scan_sums the
+// tile count of the longest scan, hist kBuckets x tiles of n (words of 4 B).
+struct CllWork
+{
+    uint32_t *runs = nullptr;
+    uint32_t *sort_keys = nullptr, *sort_vals = nullptr;
+    uint32_t *scan_sums = nullptr;
+    uint32_t *hist = nullptr;
+};
 sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *out, size_t n,
-                               uint32_t *total_host);
-sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
-sphx_status cell_linked_list_update(sphx_cell_linked_list *cll);
+                               uint32_t *total_host, uint32_t *sums = nullptr);
+sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits,
+                             const CllWork *work = nullptr);
+sphx_status cell_linked_list_update(sphx_cell_linked_list *cll, const CllWork *work = nullptr);
+// The allocation-free cell-list update: limits of its file-scope device
+// workspace, its per-device address (looked up once per device by
+// noalloc_scratch_init, with that device current) and the update itself
+constexpr uint32_t kNoAllocMaxParticles = 1u << 20;
+constexpr uint64_t kNoAllocMaxCells = 1ull << 22;
+sphx_status noalloc_scratch_init(int device);
+sphx_status cell_linked_list_update_noalloc(sphx_cell_linked_list *cll);
 sphx_status relation_to_csr(sphx_relation *rel);
 // scan = true forces the per-lane scan kernel (no LDS tile)
 sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed, bool scan);
