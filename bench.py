@@ -288,12 +288,12 @@ def main():
     ap.add_argument("--observe", type=float, default=0.01, help="observation interval of --full-run")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="instead of the throughput line: N independent 2D tanks (1k-64k particles), "
-                         "serial against batched on concurrent streams")
+                         "serial plain runs against the batch (one graph launch per advection step)")
     ap.add_argument("--served", type=int, default=0, metavar="N",
                     help="instead of the throughput line: a Poisson stream of N 3D dam-break requests "
                          "(20k-2M particles, t_end 1.0 s); p50/p99 time to solution")
-    ap.add_argument("--rate", type=float, default=0.5, help="arrival rate of --served (requests/s)")
-    ap.add_argument("--slots", type=int, default=16, help="concurrent scenes of --batched / --served")
+    ap.add_argument("--rate", type=float, default=4.0, help="arrival rate of --served (requests/s)")
+    ap.add_argument("--slots", type=int, default=256, help="concurrent scenes of --batched / --served")
     ap.add_argument("--microbench", type=int, default=0, metavar="M",
                     help="instead of the throughput line: the neighbour-search + density-summation "
                          "microbenchmark on an M^3 jittered lattice (M=256: 16.8M particles)")
@@ -316,6 +316,7 @@ def main():
 
     if args.batched > 0 or args.served > 0:
         from sphinxsys_amd import serving
+        realtime = args.served > 0
         if args.batched > 0:
             specs = serving.batched_2d_specs(args.batched)
             what = f"{args.batched} independent 2D dam-break tanks, 40 advection steps each, FP32-mixed"
@@ -328,21 +329,28 @@ def main():
             sp.geometry()  # the requests carry their particle sets
         log(f"{len(specs)} scenes generated in {time.time() - t0:.1f}s "
             f"({sum(sp.fluid_count() for sp in specs)} fluid particles)")
-        log("serial (one scene at a time, back to back)")
-        res1, wall1 = serving.Server(lib, max_active=1).run(specs, realtime=False)
+        arrivals = "arrivals in real time" if realtime else "all requests at t = 0"
+        log(f"serial: one plain run after another, {arrivals}")
+        res1, wall1 = serving.Server(lib, max_active=1).run(specs, realtime=realtime)
         serial = serving.summarize(res1, wall1)
-        log(f"  {serial['particle_steps_per_s']:.3g} particle-steps/s in {wall1:.1f}s")
-        log(f"concurrent ({args.slots} slots{', arrivals in real time' if args.served else ''})")
-        res2, wall2 = serving.Server(lib, max_active=args.slots).run(specs, realtime=args.served > 0)
+        log(f"  {serial['particle_steps_per_s']:.3g} particle-steps/s in {wall1:.1f}s, "
+            f"p50 {serial['latency_p50_s']:.2f}s p99 {serial['latency_p99_s']:.2f}s")
+        log(f"batched: continuous batching, one graph launch per advection step, {arrivals}")
+        server = serving.BatchServer(lib, max_active=args.slots)
+        res2, wall2 = server.run(specs, realtime=realtime)
         conc = serving.summarize(res2, wall2)
         log(f"  {conc['particle_steps_per_s']:.3g} particle-steps/s in {wall2:.1f}s, "
             f"p50 {conc['latency_p50_s']:.2f}s p99 {conc['latency_p99_s']:.2f}s")
         same = all(a.acoustic_steps == b.acoustic_steps and a.physical_time == b.physical_time
-                   for a, b in zip(res1, res2))
+                   and a.advection_steps == b.advection_steps for a, b in zip(res1, res2))
+        last_arrival = max(sp.arrival for sp in specs)
         print(json.dumps({"metric": "aggregate particle-steps/s and time to solution of independent scenes",
-                          "workload": what, "serial": serial, "concurrent": conc,
+                          "workload": what, "slots": args.slots, "serial": serial, "batched": conc,
                           "speedup": conc["particle_steps_per_s"] / serial["particle_steps_per_s"],
-                          "same_steps_and_time_alone_and_in_mix": same}), flush=True)
+                          "last_arrival_s": last_arrival,
+                          "arrival_bound": bool(realtime and last_arrival > 0.5 * wall2),
+                          "same_steps_and_time_alone_and_in_mix": same,
+                          "batch_tuning": server.tuning}), flush=True)
         return
     if args.microbench > 0:
         import subprocess

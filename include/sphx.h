@@ -326,6 +326,66 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
                                 const sphx_advection_config *cfg, double *physical_time,
                                 uint32_t *acoustic_steps);
 
+/* ---- batched scenes --------------------------------------------------------------
+ * No reference counterpart (the reference advances one scene per process,
+ * with one blocking launch per dynamics, implementation_sycl.h:43-92).  A
+ * batch advances many independent scenes of one device together: every stage
+ * of the advection step is one launch over the blocks of all scenes, which
+ * read their own arguments from a scene table on the device (particle and
+ * relation pointers, parameters, time-step slots); each scene's dual-time
+ * loop runs on the device from its own slots.  The step is captured once in a
+ * HIP graph (sphx_batch_options.use_graph) and replayed: one graph launch per
+ * advection step for the whole batch.  The graph holds as many acoustic
+ * sub-steps as the scenes needed last time plus a margin; a scene that needs
+ * more, whose neighbour lists outgrow their width, or that sorts this step is
+ * finished by the host through the single-scene path.  An autotuner times
+ * launch-shape candidates over the first steps (results do not depend on the
+ * shape) and keeps the fastest; sphx_batch_tuning reports what it measured.
+ *
+ * Every scene is a fluid body with its inner relation, 0..4 wall contact
+ * relations and its cell linked list, built and initialised as for
+ * sphx_advection_step, on contexts of the batch's device; all scenes share
+ * dimension, precision and Riemann solver, and use no kernel correction and
+ * no surface indication.  While a batch step runs, nothing else may use the
+ * scenes; work issued on a scene's context between batch steps must be
+ * synchronised (sphx_context_synchronize) before the next batch step.  Each
+ * scene's result is bit-identical to the same scene advanced alone with
+ * sphx_advection_step. */
+typedef struct sphx_batch sphx_batch;
+
+typedef struct sphx_batch_options {
+    int use_graph;          /* 1 (default): capture the step in a HIP graph and replay it;
+                               0: the same launches issued directly every step          */
+    int autotune;           /* 1 (default): time the shape candidates over the first
+                               steps and keep the fastest; 0: use the fields below     */
+    int tile_waves;         /* waves per workgroup of the tiled neighbour build: 1, 2, 4 */
+    uint32_t stage_min;     /* scenes with at least this many fluid particles run the
+                               acoustic passes from LDS-staged neighbour records       */
+    uint32_t substep_margin;/* captured sub-steps beyond the last step's largest count */
+} sphx_batch_options;
+
+sphx_status sphx_batch_create(int device, sphx_batch **out);
+sphx_status sphx_batch_destroy(sphx_batch *batch);
+sphx_status sphx_batch_set_options(sphx_batch *batch, const sphx_batch_options *options);
+/* adds a scene (its index is the number of scenes added before it) */
+sphx_status sphx_batch_add_scene(sphx_batch *batch, sphx_relation *inner, sphx_relation *const *walls,
+                                 int n_wall, sphx_cell_linked_list *fluid_cll,
+                                 const sphx_advection_config *cfg);
+/* removes a scene (the indices of the scenes after it drop by one); the
+ * scene's objects are untouched and may be used alone again */
+sphx_status sphx_batch_remove_scene(sphx_batch *batch, uint32_t index);
+/* One advection step of every scene (as sphx_advection_step; cfg->sort_now is
+ * replaced by sort_now[scene], null: no sort).  physical_time[scene] in/out,
+ * acoustic_steps[scene] out, status[scene] out (may be null).  A scene whose
+ * state diverged reports SPHX_E_NONFINITE, stays in the batch without
+ * advancing, and makes the call return SPHX_E_NONFINITE after the others have
+ * advanced. */
+sphx_status sphx_batch_advection_step(sphx_batch *batch, double *physical_time, uint32_t *acoustic_steps,
+                                      const int *sort_now, int *status);
+/* JSON text of the tuning (candidates, cost per particle sub-step, choice) and
+ * of the launch counts; *length receives the full length (without the NUL). */
+sphx_status sphx_batch_tuning(const sphx_batch *batch, char *json, size_t capacity, size_t *length);
+
 /* ---- instrumentation ------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around each
  * launch of the named stage (no reference counterpart; the reference times

@@ -1407,6 +1407,100 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     return SPHX_OK;
 }
 
+// batched scenes on the CPU: each scene's advection step in turn, which is
+// what the device batch must reproduce scene by scene
+struct sphx_batch
+{
+    struct Scene
+    {
+        sphx_relation *inner;
+        std::vector<sphx_relation *> walls;
+        sphx_cell_linked_list *cll;
+        sphx_advection_config cfg;
+        bool failed = false;
+    };
+    std::vector<Scene> scenes;
+    uint64_t steps = 0;
+};
+
+sphx_status sphx_batch_create(int, sphx_batch **out)
+{
+    if (!out)
+        return fail(SPHX_E_ARG, "null out");
+    *out = new sphx_batch();
+    return SPHX_OK;
+}
+sphx_status sphx_batch_destroy(sphx_batch *b)
+{
+    delete b;
+    return SPHX_OK;
+}
+sphx_status sphx_batch_set_options(sphx_batch *b, const sphx_batch_options *o)
+{
+    return b && o ? SPHX_OK : fail(SPHX_E_ARG, "null argument");
+}
+sphx_status sphx_batch_add_scene(sphx_batch *b, sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                                 sphx_cell_linked_list *cll, const sphx_advection_config *cfg)
+{
+    if (!b || !cfg || !relations_ok(inner, walls, n_wall) || !cll || cll != inner->cll)
+        return fail(SPHX_E_ARG, "batched scene arguments inconsistent");
+    sphx_batch::Scene sc{inner, std::vector<sphx_relation *>(walls, walls + n_wall), cll, *cfg};
+    b->scenes.push_back(sc);
+    return SPHX_OK;
+}
+sphx_status sphx_batch_remove_scene(sphx_batch *b, uint32_t index)
+{
+    if (!b || index >= b->scenes.size())
+        return fail(SPHX_E_ARG, "no such scene in the batch");
+    b->scenes.erase(b->scenes.begin() + index);
+    return SPHX_OK;
+}
+sphx_status sphx_batch_advection_step(sphx_batch *b, double *time, uint32_t *steps, const int *sort_now, int *status)
+{
+    if (!b || !time)
+        return fail(SPHX_E_ARG, "null argument");
+    sphx_status result = SPHX_OK;
+    for (size_t i = 0; i < b->scenes.size(); ++i)
+    {
+        sphx_batch::Scene &sc = b->scenes[i];
+        sphx_advection_config cfg = sc.cfg;
+        cfg.sort_now = sort_now ? sort_now[i] : 0;
+        uint32_t n = 0;
+        sphx_status s = sc.failed ? SPHX_E_NONFINITE
+                                  : sphx_advection_step(sc.inner, sc.walls.data(), (int)sc.walls.size(), sc.cll, &cfg,
+                                                        &time[i], &n);
+        if (s && s != SPHX_E_NONFINITE)
+            return s;
+        if (s)
+        {
+            sc.failed = true;
+            result = SPHX_E_NONFINITE;
+        }
+        if (steps)
+            steps[i] = n;
+        if (status)
+            status[i] = s;
+    }
+    b->steps++;
+    return result;
+}
+sphx_status sphx_batch_tuning(const sphx_batch *b, char *json, size_t capacity, size_t *length)
+{
+    if (!b)
+        return fail(SPHX_E_ARG, "null batch");
+    const std::string j = "{\"scenes\": " + std::to_string(b->scenes.size()) + ", \"steps\": " +
+                          std::to_string(b->steps) + ", \"serial\": true}";
+    if (length)
+        *length = j.size();
+    if (json && capacity > 0)
+    {
+        const size_t n = std::min(capacity - 1, j.size());
+        std::memcpy(json, j.data(), n);
+        json[n] = '\0';
+    }
+    return SPHX_OK;
+}
+
 // instrumentation entry points: the oracle records nothing
 sphx_status sphx_profile_enable(sphx_context *ctx, int)
 {

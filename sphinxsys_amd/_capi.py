@@ -48,6 +48,9 @@ SYMBOLS = [
     "sphx_acoustic_step_1st_half_corrected", "sphx_acoustic_step_2nd_half_corrected",
     "sphx_free_surface_indication", "sphx_viscous_force", "sphx_kernel_gradient_integral",
     "sphx_transport_velocity_correction",
+    "sphx_batch_create", "sphx_batch_destroy", "sphx_batch_set_options", "sphx_batch_add_scene",
+    "sphx_batch_remove_scene",
+    "sphx_batch_advection_step", "sphx_batch_tuning",
 ]
 
 
@@ -62,6 +65,11 @@ class AdvectionConfig(C.Structure):
                 ("acoustic_cfl", C.c_double), ("riemann", C.c_int), ("flow_type", C.c_int),
                 ("sort_now", C.c_int), ("correction", C.c_int), ("correction_alpha", C.c_double),
                 ("surface_indication", C.c_int)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("use_graph", C.c_int), ("autotune", C.c_int), ("tile_waves", C.c_int),
+                ("stage_min", C.c_uint32), ("substep_margin", C.c_uint32)]
 
 
 class SphxError(RuntimeError):
@@ -127,6 +135,13 @@ class SphxLibrary:
             "sphx_viscous_force": (C.c_int, [_vp, _pp, C.c_int, C.c_double, C.c_int]),
             "sphx_kernel_gradient_integral": (C.c_int, [_vp, _pp, C.c_int, C.c_int]),
             "sphx_transport_velocity_correction": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
+            "sphx_batch_create": (C.c_int, [C.c_int, _pp]),
+            "sphx_batch_destroy": (C.c_int, [_vp]),
+            "sphx_batch_set_options": (C.c_int, [_vp, C.POINTER(BatchOptions)]),
+            "sphx_batch_add_scene": (C.c_int, [_vp, _vp, _pp, C.c_int, _vp, C.POINTER(AdvectionConfig)]),
+            "sphx_batch_remove_scene": (C.c_int, [_vp, C.c_uint32]),
+            "sphx_batch_advection_step": (C.c_int, [_vp, _dp, _up, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "sphx_batch_tuning": (C.c_int, [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -24,6 +24,7 @@ SOURCES = {
     "config_kernels.hip": ["-ffp-contract=off"],
     "fluid_kernels.hip": [],
     "sphx_api.hip": [],
+    "batch.hip": [],
 }
 
 
@@ -39,11 +40,16 @@ def _run(cmd, verbose):
 def build(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
-    objs = []
+    # the translation units compile side by side
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = []
     for src, extra in SOURCES.items():
         obj = os.path.join(OBJ_DIR, src.replace(".hip", ".o"))
-        _run([HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append(([HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj], obj))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as pool:
+        for f in [pool.submit(_run, cmd, verbose) for cmd, _ in jobs]:
+            f.result()
+    objs = [obj for _, obj in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, *COMMON, "-shared", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)

@@ -435,3 +435,74 @@ class AdvectionStep:
         self.lib.call("sphx_advection_step", self.inner.h, self._ptrs, len(self.walls), self.cll.h,
                       C.byref(self.cfg), C.byref(t), C.byref(n))
         return t.value, n.value
+
+
+class BatchAdvection:
+    """Many scenes of one device advanced together (sphx_batch_*): one launch
+    per stage over all scenes, captured in a HIP graph and replayed once per
+    advection step; every scene's result is the one of its own AdvectionStep.
+    `scenes` are AdvectionStep objects (their inner relation, walls, cell list
+    and configuration)."""
+
+    def __init__(self, lib, scenes, device=0, options=None):
+        self.lib, self.scenes = lib, []
+        self.h = C.c_void_p()
+        lib.call("sphx_batch_create", device, C.byref(self.h))
+        if options is not None:
+            self.set_options(**options)
+        self.time = np.zeros(0)
+        self.steps = np.zeros(0, dtype=np.uint32)
+        self.status = np.zeros(0, dtype=np.int32)
+        for sc in scenes:
+            self.add(sc)
+
+    def add(self, scene, time=0.0):
+        """Admit one more scene (an AdvectionStep) at physical time `time`."""
+        self.lib.call("sphx_batch_add_scene", self.h, scene.inner.h, scene._ptrs, len(scene.walls),
+                      scene.cll.h, C.byref(scene.cfg))
+        self.scenes.append(scene)
+        self.time = np.append(self.time, time)
+        self.steps = np.append(self.steps, np.uint32(0))
+        self.status = np.append(self.status, np.int32(0))
+
+    def remove(self, index):
+        """Release scene `index` from the batch (later indices drop by one)."""
+        self.lib.call("sphx_batch_remove_scene", self.h, int(index))
+        del self.scenes[index]
+        self.time = np.delete(self.time, index)
+        self.steps = np.delete(self.steps, index)
+        self.status = np.delete(self.status, index)
+
+    def set_options(self, use_graph=1, autotune=1, tile_waves=1, stage_min=0, substep_margin=1):
+        o = capi.BatchOptions(use_graph, autotune, tile_waves, stage_min, substep_margin)
+        self.lib.call("sphx_batch_set_options", self.h, C.byref(o))
+
+    def exec(self, sort_now=None):
+        """One advection step of every scene; returns the per-scene statuses
+        (0, or SPHX_E_NONFINITE for a diverged scene)."""
+        if not self.scenes:
+            return self.status.copy()
+        sort = None
+        if sort_now is not None:
+            flags = np.ascontiguousarray(sort_now, dtype=np.int32)
+            sort = flags.ctypes.data_as(C.POINTER(C.c_int))
+        st = self.lib.lib.sphx_batch_advection_step(
+            self.h, self.time.ctypes.data_as(C.POINTER(C.c_double)),
+            self.steps.ctypes.data_as(C.POINTER(C.c_uint32)), sort,
+            self.status.ctypes.data_as(C.POINTER(C.c_int)))
+        if st not in (0, 4):  # SPHX_E_NONFINITE is reported per scene
+            self.lib.check(st, "sphx_batch_advection_step")
+        return self.status.copy()
+
+    def tuning(self):
+        import json
+        n = C.c_size_t()
+        self.lib.call("sphx_batch_tuning", self.h, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        self.lib.call("sphx_batch_tuning", self.h, buf, n.value + 1, C.byref(n))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self.h:
+            self.lib.lib.sphx_batch_destroy(self.h)
+            self.h = C.c_void_p()

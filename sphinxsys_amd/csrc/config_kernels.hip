@@ -9,6 +9,7 @@
 
 #include "sphx_internal.hpp"
 
+#include <cstring>
 #include <mutex>
 
 namespace sphx
@@ -62,9 +63,9 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
 //---------------------------------------------------------------------------
 // exclusive scan: tile reduce -> scan of tile sums -> tile scan
 //---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t *in, size_t n, uint32_t *sums)
+__device__ __forceinline__ void scan_reduce_body(const uint32_t *in, size_t n, uint32_t *sums, uint32_t blk)
 {
-    size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
+    size_t base = (size_t)blk * kTile + (size_t)threadIdx.x * kItems;
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
@@ -73,12 +74,15 @@ __global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t *in, si
     uint32_t tot;
     block_exclusive_scan(s, &tot);
     if (threadIdx.x == 0)
-        sums[blockIdx.x] = tot;
+        sums[blk] = tot;
+}
+__global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t *in, size_t n, uint32_t *sums)
+{
+    scan_reduce_body(in, n, sums, blockIdx.x);
 }
 
 // single workgroup: exclusive scan of the tile sums in place; grand total to *total
-__global__ void __launch_bounds__(kThreads) k_scan_sums(uint32_t *sums, uint32_t nsums,
-                                                        unsigned long long *total)
+__device__ __forceinline__ void scan_sums_body(uint32_t *sums, uint32_t nsums, unsigned long long *total)
 {
     uint32_t carry = 0;
     for (uint32_t start = 0; start < nsums; start += kTile)
@@ -108,11 +112,16 @@ __global__ void __launch_bounds__(kThreads) k_scan_sums(uint32_t *sums, uint32_t
     if (threadIdx.x == 0 && total)
         *total = carry;
 }
-
-__global__ void __launch_bounds__(kThreads) k_scan_apply(const uint32_t *in, uint32_t *out, size_t n,
-                                                         const uint32_t *sums)
+__global__ void __launch_bounds__(kThreads) k_scan_sums(uint32_t *sums, uint32_t nsums,
+                                                        unsigned long long *total)
 {
-    size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
+    scan_sums_body(sums, nsums, total);
+}
+
+__device__ __forceinline__ void scan_apply_body(const uint32_t *in, uint32_t *out, size_t n, const uint32_t *sums,
+                                                uint32_t blk)
+{
+    size_t base = (size_t)blk * kTile + (size_t)threadIdx.x * kItems;
     uint32_t v[kItems];
     uint32_t s = 0;
 #pragma unroll
@@ -122,7 +131,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_apply(const uint32_t *in, uin
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = sums[blockIdx.x] + block_exclusive_scan(s, &tot);
+    uint32_t run = sums[blk] + block_exclusive_scan(s, &tot);
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
         if (base + k < n)
@@ -130,6 +139,11 @@ __global__ void __launch_bounds__(kThreads) k_scan_apply(const uint32_t *in, uin
             out[base + k] = run;
             run += v[k];
         }
+}
+__global__ void __launch_bounds__(kThreads) k_scan_apply(const uint32_t *in, uint32_t *out, size_t n,
+                                                         const uint32_t *sums)
+{
+    scan_apply_body(in, out, n, sums, blockIdx.x);
 }
 
 // the whole scan of at most one tile in one launch; the total to *total
@@ -286,10 +300,10 @@ __device__ __forceinline__ void load_pos(const PosPtrs &p, uint32_t i, double *x
 }
 
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_cell_keys(PosPtrs pos, uint32_t n, MeshParams m, uint32_t *keys,
-                                                        uint32_t *counts)
+__device__ __forceinline__ void cell_keys_body(const PosPtrs &pos, uint32_t n, const MeshParams &m, uint32_t *keys,
+                                               uint32_t *counts, uint32_t blk)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x, lane = threadIdx.x & 63u;
+    const uint32_t i = blk * kThreads + threadIdx.x, lane = threadIdx.x & 63u;
     uint32_t lin = 0xffffffffu;
     if (i < n)
     {
@@ -309,6 +323,12 @@ __global__ void __launch_bounds__(kThreads) k_cell_keys(PosPtrs pos, uint32_t n,
     const uint32_t next = above ? (uint32_t)__ffsll((long long)above) - 1u : 64u;
     if (head && i < n)
         atomicAdd(&counts[lin], next - lane);
+}
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_cell_keys(PosPtrs pos, uint32_t n, MeshParams m, uint32_t *keys,
+                                                        uint32_t *counts)
+{
+    cell_keys_body<D>(pos, n, m, keys, counts, blockIdx.x);
 }
 
 // runs of equal cell keys in particle order: flag[i] = 1 where a run starts
@@ -758,17 +778,16 @@ __device__ __forceinline__ uint32_t pick(const uint32_t (&v)[R], uint32_t idx)
     return out;
 }
 
-template <int D>
-__global__ void __launch_bounds__(kTileWaves * 64) k_relation_tile(BuildArgs b)
+template <int D, int W>
+__device__ __forceinline__ void relation_tile_body(const BuildArgs &b, const uint32_t blk)
 {
     constexpr int R = D == 3 ? 9 : 3; // cell rows of the search box, l -> m order
     constexpr int QMAX = 8;           // candidates of a cell kept in registers: 64 * QMAX
     const RelationArgs &a = b.r;
-    __shared__ unsigned long long masks[kTileWaves][64][kMaskSlots];
+    __shared__ unsigned long long masks[W][64][kMaskSlots];
     __shared__ uint32_t any_dense;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    const uint32_t blk = xcd_block_index();
-    const uint32_t chunk = blk * kTileWaves + wave;
+    const uint32_t chunk = blk * W + wave;
     const uint32_t i = chunk * 64u + lane;
     const bool valid = i < a.n_src;
     const uint32_t kmax = b.kmax;
@@ -1074,6 +1093,12 @@ __global__ void __launch_bounds__(kTileWaves * 64) k_relation_tile(BuildArgs b)
     }
 }
 
+template <int D>
+__global__ void __launch_bounds__(kTileWaves * 64) k_relation_tile(BuildArgs b)
+{
+    relation_tile_body<D, kTileWaves>(b, xcd_block_index());
+}
+
 //---------------------------------------------------------------------------
 // staging sets of the inner fluid relation, from its finished slices: for each
 // block of kStageBlock source particles the distinct neighbours (stage_keys,
@@ -1112,14 +1137,13 @@ __device__ __forceinline__ uint32_t seg_find(const uint32_t *owner, uint32_t gro
     return s;
 }
 
-__global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
+__device__ __forceinline__ void stage_sets_body(const StageArgs &s, const uint32_t blk)
 {
     __shared__ uint32_t owner[kNumSeg];
     __shared__ uint32_t mask[kNumSeg];
     __shared__ uint32_t seg_base[kNumSeg];
     __shared__ uint32_t seg_full;
     const uint32_t t = threadIdx.x;
-    const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + t;
     const bool valid = i < s.n_src;
     for (uint32_t k = t; k < kNumSeg; k += kThreads)
@@ -1236,6 +1260,162 @@ __global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
     }
     if (t == 0)
         s.stage_count[blk] = total;
+}
+__global__ void __launch_bounds__(kThreads) k_stage_sets(StageArgs s)
+{
+    stage_sets_body(s, xcd_block_index());
+}
+
+//---------------------------------------------------------------------------
+// Batched configuration update over the scenes of a batch (map entries:
+// scene or relation << 16 | block of it).  The cell list is a counting sort:
+// per-cell counts, a scan, an atomic scatter and then each cell's indices
+// put in ascending order, which is the order of the run-based update above
+// and of the reference's canonical cell list.
+//---------------------------------------------------------------------------
+struct CllScene
+{
+    PosPtrs pos;
+    uint32_t n;
+    uint32_t ncells;
+    MeshParams m;
+    uint32_t *keys, *cell_offset, *particle_index;
+    uint32_t *cursor;    // ncells
+    uint32_t *tile_sums; // scan tiles of ncells + 1
+    uint32_t ntiles;
+};
+
+__device__ __forceinline__ uint32_t bmap_owner(uint32_t m) { return m >> 16; }
+__device__ __forceinline__ uint32_t bmap_block(uint32_t m) { return m & 0xffffu; }
+
+__global__ void __launch_bounds__(kThreads) kb_cll_clear(const CllScene *__restrict__ tab,
+                                                         const uint32_t *__restrict__ map,
+                                                         const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
+    if (!gate[sc])
+        return;
+    const CllScene &a = tab[sc];
+    const uint32_t c = bmap_block(m) * kThreads + threadIdx.x;
+    if (c <= a.ncells)
+        a.cell_offset[c] = 0u;
+    if (c < a.ncells)
+        a.cursor[c] = 0u;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kThreads) kb_cll_count(const CllScene *__restrict__ tab,
+                                                         const uint32_t *__restrict__ map,
+                                                         const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
+    if (!gate[sc])
+        return;
+    const CllScene &a = tab[sc];
+    cell_keys_body<D>(a.pos, a.n, a.m, a.keys, a.cell_offset, bmap_block(m));
+}
+
+__global__ void __launch_bounds__(kThreads) kb_cll_scan_reduce(const CllScene *__restrict__ tab,
+                                                               const uint32_t *__restrict__ map,
+                                                               const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
+    if (!gate[sc])
+        return;
+    const CllScene &a = tab[sc];
+    scan_reduce_body(a.cell_offset, (size_t)a.ncells + 1, a.tile_sums, bmap_block(m));
+}
+
+// one workgroup per scene
+__global__ void __launch_bounds__(kThreads) kb_cll_scan_sums(const CllScene *__restrict__ tab,
+                                                             const uint32_t *__restrict__ gate)
+{
+    if (!gate[blockIdx.x])
+        return;
+    const CllScene &a = tab[blockIdx.x];
+    scan_sums_body(a.tile_sums, a.ntiles, nullptr);
+}
+
+__global__ void __launch_bounds__(kThreads) kb_cll_scan_apply(const CllScene *__restrict__ tab,
+                                                              const uint32_t *__restrict__ map,
+                                                              const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
+    if (!gate[sc])
+        return;
+    const CllScene &a = tab[sc];
+    scan_apply_body(a.cell_offset, a.cell_offset, (size_t)a.ncells + 1, a.tile_sums, bmap_block(m));
+}
+
+__global__ void __launch_bounds__(kThreads) kb_cll_scatter(const CllScene *__restrict__ tab,
+                                                           const uint32_t *__restrict__ map,
+                                                           const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
+    if (!gate[sc])
+        return;
+    const CllScene &a = tab[sc];
+    const uint32_t i = bmap_block(m) * kThreads + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const uint32_t key = a.keys[i];
+    a.particle_index[a.cell_offset[key] + atomicAdd(&a.cursor[key], 1u)] = i;
+}
+
+// the indices of each cell in ascending order (cells hold a few particles)
+__global__ void __launch_bounds__(kThreads) kb_cll_order(const CllScene *__restrict__ tab,
+                                                         const uint32_t *__restrict__ map,
+                                                         const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
+    if (!gate[sc])
+        return;
+    const CllScene &a = tab[sc];
+    const uint32_t c = bmap_block(m) * kThreads + threadIdx.x;
+    if (c >= a.ncells)
+        return;
+    const uint32_t lo = a.cell_offset[c], hi = a.cell_offset[c + 1];
+    uint32_t *idx = a.particle_index;
+    for (uint32_t q = lo + 1; q < hi; ++q)
+    {
+        const uint32_t v = idx[q];
+        uint32_t r = q;
+        while (r > lo && idx[r - 1] > v)
+        {
+            idx[r] = idx[r - 1];
+            --r;
+        }
+        idx[r] = v;
+    }
+}
+
+__global__ void kb_zero_u64(unsigned long long *p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0ull;
+}
+
+template <int D, int W>
+__global__ void __launch_bounds__(W * 64) kb_relation_tile(const BuildArgs *__restrict__ tab,
+                                                           const uint32_t *__restrict__ map,
+                                                           const uint32_t *__restrict__ rel_scene,
+                                                           const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[xcd_block_index()], r = bmap_owner(m);
+    if (!gate[rel_scene[r]])
+        return;
+    relation_tile_body<D, W>(tab[r], bmap_block(m));
+}
+
+__global__ void __launch_bounds__(kThreads) kb_stage_sets(const StageArgs *__restrict__ tab,
+                                                          const uint32_t *__restrict__ map,
+                                                          const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[xcd_block_index()], sc = bmap_owner(m);
+    if (!gate[sc] || tab[sc].n_src == 0u)
+        return;
+    stage_sets_body(tab[sc], bmap_block(m));
 }
 
 __global__ void k_chunk_stride(uint32_t *chunk, uint32_t nchunks, uint32_t stride)
@@ -1714,6 +1894,106 @@ sphx_status sorted_id_update(sphx_body *b)
         return SPHX_OK;
     hipLaunchKernelGGL(k_sorted_id, dim3(grid_for(b->n, 256)), dim3(256), 0, b->ctx->stream, b->original_id,
                        b->sorted_id, b->n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+} // namespace sphx
+
+//=============================================================================
+// batched configuration update (host side)
+//=============================================================================
+namespace sphx
+{
+size_t batch_cll_args_bytes() { return sizeof(CllScene); }
+uint32_t batch_cll_tiles(const sphx_cell_linked_list *cll) { return grid_for(cll->ncells + 1, kTile); }
+
+void batch_cll_args(void *dst, sphx_cell_linked_list *cll, uint32_t *cursor, uint32_t *tile_sums)
+{
+    CllScene a{};
+    a.pos = pos_ptrs(cll->body);
+    a.n = cll->body->n;
+    a.ncells = (uint32_t)cll->ncells;
+    a.m = cll->mesh;
+    a.keys = cll->keys;
+    a.cell_offset = cll->cell_offset;
+    a.particle_index = cll->particle_index;
+    a.cursor = cursor;
+    a.tile_sums = tile_sums;
+    a.ntiles = batch_cll_tiles(cll);
+    std::memcpy(dst, &a, sizeof(a));
+}
+
+size_t batch_build_args_bytes() { return sizeof(BuildArgs); }
+
+void batch_build_args(void *dst, sphx_relation *rel, unsigned long long *needed)
+{
+    BuildArgs b;
+    b.r = relation_args(rel);
+    b.kmax = rel->kmax;
+    b.needed = needed;
+    std::memcpy(dst, &b, sizeof(b));
+}
+
+size_t batch_stage_args_bytes() { return sizeof(StageArgs); }
+
+void batch_stage_args(void *dst, sphx_relation *rel)
+{
+    StageArgs a{};
+    if (rel && rel->staged && rel->source->n > 0)
+    {
+        a.n_src = rel->source->n;
+        a.cap = rel->stage_cap;
+        a.neighbor_size = rel->neighbor_size;
+        a.chunk_offset = rel->chunk_offset;
+        a.slice = rel->slice;
+        a.stage_count = rel->stage_count;
+        a.stage_keys = rel->stage_keys;
+        a.local_slice = rel->local_slice;
+    }
+    std::memcpy(dst, &a, sizeof(a));
+}
+
+sphx_status batch_cell_linked_list(const BatchConfig &b)
+{
+    const CllScene *tab = (const CllScene *)b.cll;
+    const dim3 blk(kThreads);
+    hipLaunchKernelGGL(kb_cll_clear, dim3(b.blocks_cells), blk, 0, b.stream, tab, b.map_cells, b.gate);
+    if (b.D == 2)
+        hipLaunchKernelGGL(kb_cll_count<2>, dim3(b.blocks_particles), blk, 0, b.stream, tab, b.map_particles, b.gate);
+    else
+        hipLaunchKernelGGL(kb_cll_count<3>, dim3(b.blocks_particles), blk, 0, b.stream, tab, b.map_particles, b.gate);
+    hipLaunchKernelGGL(kb_cll_scan_reduce, dim3(b.blocks_tiles), blk, 0, b.stream, tab, b.map_tiles, b.gate);
+    hipLaunchKernelGGL(kb_cll_scan_sums, dim3(b.n_scenes), blk, 0, b.stream, tab, b.gate);
+    hipLaunchKernelGGL(kb_cll_scan_apply, dim3(b.blocks_tiles), blk, 0, b.stream, tab, b.map_tiles, b.gate);
+    hipLaunchKernelGGL(kb_cll_scatter, dim3(b.blocks_particles), blk, 0, b.stream, tab, b.map_particles, b.gate);
+    hipLaunchKernelGGL(kb_cll_order, dim3(b.blocks_cells), blk, 0, b.stream, tab, b.map_cells, b.gate);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+template <int D, int W>
+static void launch_batch_tile(const BatchConfig &b)
+{
+    hipLaunchKernelGGL((kb_relation_tile<D, W>), dim3(b.blocks_chunks), dim3(W * 64), 0, b.stream,
+                       (const BuildArgs *)b.build, b.map_chunks, b.rel_scene, b.gate);
+}
+
+sphx_status batch_relations(const BatchConfig &b)
+{
+    hipLaunchKernelGGL(kb_zero_u64, dim3(grid_for(b.n_rel, 64)), dim3(64), 0, b.stream, b.needed, b.n_rel);
+    const int w = b.tile_waves;
+    if (w != 1 && w != 2 && w != 4)
+    {
+        set_error("tiled relation build: 1, 2 or 4 waves per workgroup");
+        return SPHX_E_ARG;
+    }
+    if (b.D == 2)
+        w == 1 ? launch_batch_tile<2, 1>(b) : w == 2 ? launch_batch_tile<2, 2>(b) : launch_batch_tile<2, 4>(b);
+    else
+        w == 1 ? launch_batch_tile<3, 1>(b) : w == 2 ? launch_batch_tile<3, 2>(b) : launch_batch_tile<3, 4>(b);
+    if (b.stage)
+        hipLaunchKernelGGL(kb_stage_sets, dim3(b.blocks_particles), dim3(kThreads), 0, b.stream,
+                           (const StageArgs *)b.stage, b.map_particles, b.gate);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

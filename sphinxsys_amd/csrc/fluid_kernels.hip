@@ -17,6 +17,7 @@
 #include "sphx_internal.hpp"
 
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 namespace sphx
@@ -97,6 +98,8 @@ struct FluidArgs
     // device-side sub-step control (null: dt is the kernel argument): the
     // acoustic kernels return at once when STEP_ACTIVE is 0 and take STEP_DT
     const unsigned long long *step;
+    // max lanes of the time-step reductions (batched launches: each scene's own)
+    unsigned long long *lanes;
 };
 
 // dt of an acoustic kernel: the argument, or the device-side sub-step's; false
@@ -294,10 +297,10 @@ __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, 
 // AcousticStep1stHalf::InitializeKernel (acoustic_step_1st_half.hpp:66-74)
 //---------------------------------------------------------------------------
 template <int D, int P>
-__global__ void __launch_bounds__(kThreads) k_init_1st(FluidArgs<typename Prec<P>::T> a, double dt_d)
+__device__ __forceinline__ void init_1st_body(const FluidArgs<typename Prec<P>::T> &a, uint32_t blk, double dt_d)
 {
     using T = typename Prec<P>::T;
-    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t i = blk * kThreads + threadIdx.x;
     if (i >= a.n || !step_dt(a, dt_d))
         return;
     const T dt = T(dt_d);
@@ -311,6 +314,12 @@ __global__ void __launch_bounds__(kThreads) k_init_1st(FluidArgs<typename Prec<P
     a.dpos[1][i] += (double)(v.y * dt * T(0.5));
     if (D == 3)
         a.dpos[2][i] += (double)(v.z * dt * T(0.5));
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_init_1st(FluidArgs<typename Prec<P>::T> a, double dt_d)
+{
+    init_1st_body<D, P>(a, blockIdx.x, dt_d);
 }
 
 //---------------------------------------------------------------------------
@@ -370,10 +379,10 @@ struct Mat3
 // the wall term as B_i sum_w (p_i + p_w) dW V_w e.
 //---------------------------------------------------------------------------
 template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
-__global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
+__device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> &a, const uint32_t blk, double dt_d)
 {
     using T = typename Prec<P>::T;
-    if (!step_dt(a, dt_d)) // grid-uniform: before any barrier
+    if (!step_dt(a, dt_d)) // block-uniform: before any barrier
         return;
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
@@ -381,7 +390,6 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     stage_poly(poly_in, a.poly, true);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, true);
-    const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and p of the block's distinct neighbours
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
@@ -520,6 +528,12 @@ __global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::
     }
 }
 
+template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
+__global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
+{
+    pass1_body<D, P, RIEMANN, FUSE_INIT_2ND, CORR>(a, xcd_block_index(), dt_d);
+}
+
 // DissipativePJump of the Riemann solvers divided by rho0c0_geo_ave (the
 // factor is applied once per particle): u times the limiter of
 // AcousticRiemannSolverCK, TruncatedLinear(3) on inv_c0 * max(u, 0), or 1
@@ -546,11 +560,11 @@ static_assert(kPass2Cap <= kStageCap, "pass 2 stages a prefix of the staging cap
 // FP32-mixed: registers capped for 4 waves per SIMD, which its 40 KB of LDS
 // per block allows; FP64 (not staged) keeps its registers
 template <int D, int P, bool RIEMANN, bool FUSE_CFL, bool CORR>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 4 : 1)))
-k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_slot)
+__device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> &a, const uint32_t blk, double dt_d,
+                                           unsigned long long *max_slot)
 {
     using T = typename Prec<P>::T;
-    if (!step_dt(a, dt_d)) // grid-uniform: before any barrier
+    if (!step_dt(a, dt_d)) // block-uniform: before any barrier
         return;
     __shared__ PolyLDS<T> poly_in;
     __shared__ PolyLDS<T> poly_w[kMaxWalls];
@@ -559,7 +573,6 @@ k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_s
     stage_poly(poly_in, a.poly, true);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, true);
-    const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and the velocity components
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
@@ -727,11 +740,18 @@ k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_s
     }
 }
 
+template <int D, int P, bool RIEMANN, bool FUSE_CFL, bool CORR>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 4 : 1)))
+k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_slot)
+{
+    pass2_body<D, P, RIEMANN, FUSE_CFL, CORR>(a, xcd_block_index(), dt_d, max_slot);
+}
+
 //---------------------------------------------------------------------------
 // CompressionSummation<Inner<>> + <Contact<>> (density_regularization.hpp:40-83)
 //---------------------------------------------------------------------------
 template <int D, int P>
-__global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<P>::T> a)
+__device__ __forceinline__ void summation_body(const FluidArgs<typename Prec<P>::T> &a, const uint32_t blk)
 {
     using T = typename Prec<P>::T;
     __shared__ PolyLDS<T> poly_in;
@@ -740,7 +760,6 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
     stage_poly(poly_in, a.poly, false);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, false);
-    const uint32_t blk = xcd_block_index();
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z, Vol_ref) of the block's distinct neighbours
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
@@ -798,6 +817,12 @@ __global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<
         }
     }
     a.S[i] = s;
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<P>::T> a)
+{
+    summation_body<D, P>(a, xcd_block_index());
 }
 
 //---------------------------------------------------------------------------
@@ -1269,10 +1294,9 @@ __global__ void k_transport_correction(FluidArgs<typename Prec<P>::T> a, double 
 // state dynamics
 //---------------------------------------------------------------------------
 template <int D, int P>
-__global__ void k_regularize(FluidArgs<typename Prec<P>::T> a, int flow)
+__device__ __forceinline__ void regularize_body(const FluidArgs<typename Prec<P>::T> &a, uint32_t i, int flow)
 {
     using T = typename Prec<P>::T;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n)
         return;
     // Regularization<FreeSurface>: C = max(S, 1); <Internal>: C = S
@@ -1280,6 +1304,12 @@ __global__ void k_regularize(FluidArgs<typename Prec<P>::T> a, int flow)
     T c = flow == SPHX_FLOW_FREE_SURFACE ? dmax(S, T(1)) : S;
     a.C[i] = Prec<P>::c_minus_one ? c - T(1) : c;
     a.rho[i] = c * a.rho0;
+}
+
+template <int D, int P>
+__global__ void k_regularize(FluidArgs<typename Prec<P>::T> a, int flow)
+{
+    regularize_body<D, P>(a, blockIdx.x * blockDim.x + threadIdx.x, flow);
 }
 
 struct PosArgs
@@ -1291,11 +1321,10 @@ struct PosArgs
 // AdvectionStepSetup (fluid_time_step_ck.h:148-152), fused with the refresh
 // of the (x, y, z, Vol) gather record
 template <int D, int P>
-__global__ void k_setup(FluidArgs<typename Prec<P>::T> a, PosArgs pa, typename Prec<P>::T *Vol,
-                        Rec4<typename Prec<P>::T> *posvol)
+__device__ __forceinline__ void setup_body(const FluidArgs<typename Prec<P>::T> &a, const PosArgs &pa,
+                                           typename Prec<P>::T *Vol, Rec4<typename Prec<P>::T> *posvol, uint32_t i)
 {
     using T = typename Prec<P>::T;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n)
         return;
     T vol = a.mass[i] / a.rho[i];
@@ -1319,11 +1348,18 @@ __global__ void k_setup(FluidArgs<typename Prec<P>::T> a, PosArgs pa, typename P
     pa.dpos[1][i] = 0.0;
 }
 
+template <int D, int P>
+__global__ void k_setup(FluidArgs<typename Prec<P>::T> a, PosArgs pa, typename Prec<P>::T *Vol,
+                        Rec4<typename Prec<P>::T> *posvol)
+{
+    setup_body<D, P>(a, pa, Vol, posvol, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // UpdateParticlePosition (fluid_time_step_ck.h:176-179) + record refresh
 template <int D, class T>
-__global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol, Rec4<float> *pos_f32)
+__device__ __forceinline__ void position_body(const PosArgs &pa, const T *Vol, Rec4<T> *posvol, Rec4<float> *pos_f32,
+                                              uint32_t i)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pa.n)
         return;
     Rec4<T> r;
@@ -1348,6 +1384,12 @@ __global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol, Rec
     posvol[i] = r;
     if (pos_f32 != (Rec4<float> *)posvol)
         pos_f32[i] = Rec4<float>{(float)r.x, (float)r.y, D == 3 ? (float)r.z : 0.0f, 0.0f};
+}
+
+template <int D, class T>
+__global__ void k_update_position(PosArgs pa, const T *Vol, Rec4<T> *posvol, Rec4<float> *pos_f32)
+{
+    position_body<D, T>(pa, Vol, posvol, pos_f32, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 template <int D, class T>
@@ -1407,11 +1449,11 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long *slot)
 }
 
 template <int D, int P>
-__global__ void __launch_bounds__(kThreads) k_signal_max(FluidArgs<typename Prec<P>::T> a,
-                                                         unsigned long long *slot)
+__device__ __forceinline__ void signal_body(const FluidArgs<typename Prec<P>::T> &a, uint32_t blk,
+                                            unsigned long long *slot)
 {
     using T = typename Prec<P>::T;
-    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t i = blk * kThreads + threadIdx.x;
     double v = -1.0;
     if (i < a.n)
     {
@@ -1428,11 +1470,18 @@ __global__ void __launch_bounds__(kThreads) k_signal_max(FluidArgs<typename Prec
 }
 
 template <int D, int P>
-__global__ void __launch_bounds__(kThreads) k_speed_max(FluidArgs<typename Prec<P>::T> a,
-                                                        unsigned long long *slot)
+__global__ void __launch_bounds__(kThreads) k_signal_max(FluidArgs<typename Prec<P>::T> a,
+                                                         unsigned long long *slot)
+{
+    signal_body<D, P>(a, blockIdx.x, slot);
+}
+
+template <int D, int P>
+__device__ __forceinline__ void speed_body(const FluidArgs<typename Prec<P>::T> &a, uint32_t blk,
+                                           unsigned long long *slot)
 {
     using T = typename Prec<P>::T;
-    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t i = blk * kThreads + threadIdx.x;
     double v = -1.0;
     if (i < a.n)
     {
@@ -1441,6 +1490,13 @@ __global__ void __launch_bounds__(kThreads) k_speed_max(FluidArgs<typename Prec<
         v = isfinite(v2) ? (double)v2 : __builtin_huge_val();
     }
     block_max_to(v, slot);
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) k_speed_max(FluidArgs<typename Prec<P>::T> a,
+                                                        unsigned long long *slot)
+{
+    speed_body<D, P>(a, blockIdx.x, slot);
 }
 
 __global__ void __launch_bounds__(kThreads) k_axis_max(const double *x, uint32_t n, unsigned long long *slot)
@@ -1564,10 +1620,9 @@ __device__ __forceinline__ double get_double(const unsigned long long *slot)
     return __longlong_as_double((long long)*slot);
 }
 
-__global__ void k_step_begin(unsigned long long *scalars, double u_ref, double cfl, double h, double time)
+__device__ __forceinline__ void step_begin_body(unsigned long long *scalars, double u_ref, double cfl, double h,
+                                                double time)
 {
-    if (threadIdx.x != 0)
-        return;
     unsigned long long *st = scalars + SLOT_STEP;
     const double m = lanes_max_reset(scalars + SLOT_MAX);
     // AdvectionTimeStepCK::FinishDynamics::Result, fluid_time_step_ck.cpp:23-26
@@ -1582,10 +1637,14 @@ __global__ void k_step_begin(unsigned long long *scalars, double u_ref, double c
     st[STEP_ERROR] = isfinite(adv_dt) && adv_dt > 0.0 ? 0ull : 1ull;
 }
 
-__global__ void k_substep_begin(unsigned long long *scalars, double cfl, double h)
+__global__ void k_step_begin(unsigned long long *scalars, double u_ref, double cfl, double h, double time)
 {
-    if (threadIdx.x != 0)
-        return;
+    if (threadIdx.x == 0)
+        step_begin_body(scalars, u_ref, cfl, h, time);
+}
+
+__device__ __forceinline__ void substep_begin_body(unsigned long long *scalars, double cfl, double h)
+{
     unsigned long long *st = scalars + SLOT_STEP;
     const double relaxation = get_double(st + STEP_RELAXATION);
     if (st[STEP_ERROR] != 0 || !(relaxation < get_double(st + STEP_ADV_DT)))
@@ -1606,6 +1665,157 @@ __global__ void k_substep_begin(unsigned long long *scalars, double cfl, double 
     put_double(st + STEP_TIME, get_double(st + STEP_TIME) + dt);
     st[STEP_COUNT] += 1;
     st[STEP_ACTIVE] = 1;
+}
+
+__global__ void k_substep_begin(unsigned long long *scalars, double cfl, double h)
+{
+    if (threadIdx.x == 0)
+        substep_begin_body(scalars, cfl, h);
+}
+
+//---------------------------------------------------------------------------
+// Batched launches: one grid over the blocks of every scene of a batch.
+// map[b] = scene << 16 | block of that scene, laid out scene after scene, so
+// that the XCD-aware block order keeps each scene's blocks on one XCD where
+// it can; every block reads its scene's arguments from the scene table.
+// The tail kernels of an advection step run only for the scenes whose gate
+// is set (their sub-steps are done and the tail is not left to the host).
+//---------------------------------------------------------------------------
+template <class T>
+struct SceneArgs
+{
+    FluidArgs<T> f;
+    PosArgs pa;
+    T *Vol;
+    Rec4<T> *posvol;
+    Rec4<float> *pos_f32;
+    unsigned long long *scalars; // the scene context's scalar slots (step state, max lanes)
+    const double *time;          // physical time at the start of the step
+    double u_ref, advection_cfl, acoustic_cfl, h;
+    int flow;
+};
+
+__device__ __forceinline__ uint32_t map_scene(uint32_t m) { return m >> 16; }
+__device__ __forceinline__ uint32_t map_block(uint32_t m) { return m & 0xffffu; }
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) kb_summation(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                         const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[xcd_block_index()];
+    summation_body<D, P>(tab[map_scene(m)].f, map_block(m));
+}
+
+// DensityRegularization + AdvectionStepSetup, which touch each particle alone
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) kb_regularize_setup(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                                const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[blockIdx.x];
+    const SceneArgs<typename Prec<P>::T> &sa = tab[map_scene(m)];
+    const uint32_t i = map_block(m) * kThreads + threadIdx.x;
+    regularize_body<D, P>(sa.f, i, sa.flow);
+    setup_body<D, P>(sa.f, sa.pa, sa.Vol, sa.posvol, i);
+}
+
+// the max lanes of every scene to the lowest value (one block per scene)
+template <class T>
+__global__ void kb_reset_lanes(const SceneArgs<T> *__restrict__ tab, uint32_t n_scenes)
+{
+    if (blockIdx.x < n_scenes && threadIdx.x < (unsigned)kMaxLanes)
+        tab[blockIdx.x].scalars[SLOT_MAX + threadIdx.x * kMaxStride] = kOrderedLowest;
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) kb_speed_max(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                         const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[blockIdx.x];
+    const SceneArgs<typename Prec<P>::T> &sa = tab[map_scene(m)];
+    speed_body<D, P>(sa.f, map_block(m), sa.scalars + SLOT_MAX);
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) kb_signal_max(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                          const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[blockIdx.x];
+    const SceneArgs<typename Prec<P>::T> &sa = tab[map_scene(m)];
+    signal_body<D, P>(sa.f, map_block(m), sa.scalars + SLOT_MAX);
+}
+
+// one thread per scene
+template <class T>
+__global__ void kb_step_begin(const SceneArgs<T> *__restrict__ tab, uint32_t n_scenes)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_scenes)
+    {
+        const SceneArgs<T> &sa = tab[s];
+        step_begin_body(sa.scalars, sa.u_ref, sa.advection_cfl, sa.h, *sa.time);
+    }
+}
+template <class T>
+__global__ void kb_substep_begin(const SceneArgs<T> *__restrict__ tab, uint32_t n_scenes)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_scenes)
+        substep_begin_body(tab[s].scalars, tab[s].acoustic_cfl, tab[s].h);
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) kb_init_1st(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                        const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[blockIdx.x];
+    init_1st_body<D, P>(tab[map_scene(m)].f, map_block(m), 0.0);
+}
+
+template <int D, int P, bool RIEMANN>
+__global__ void __launch_bounds__(kThreads) kb_pass1(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                     const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[xcd_block_index()];
+    pass1_body<D, P, RIEMANN, true, false>(tab[map_scene(m)].f, map_block(m), 0.0);
+}
+
+template <int D, int P, bool RIEMANN>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 4 : 1)))
+kb_pass2(const SceneArgs<typename Prec<P>::T> *__restrict__ tab, const uint32_t *__restrict__ map)
+{
+    const uint32_t m = map[xcd_block_index()];
+    const SceneArgs<typename Prec<P>::T> &sa = tab[map_scene(m)];
+    pass2_body<D, P, RIEMANN, true, false>(sa.f, map_block(m), 0.0, sa.scalars + SLOT_MAX);
+}
+
+// after the captured sub-steps: gate[s] = 1 when scene s finished its loop
+// without error and its tail is run on the device (tail_on[s]); its step
+// state is copied to state[s * STEP_SLOTS ...] for the host
+template <class T>
+__global__ void kb_gate(const SceneArgs<T> *__restrict__ tab, uint32_t n_scenes, const uint32_t *__restrict__ tail_on,
+                        uint32_t *gate, unsigned long long *state)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_scenes)
+        return;
+    const unsigned long long *st = tab[s].scalars + SLOT_STEP;
+    for (int k = 0; k < STEP_SLOTS; ++k)
+        state[(size_t)s * STEP_SLOTS + k] = st[k];
+    const bool done = st[STEP_ERROR] == 0 && !(get_double(st + STEP_RELAXATION) < get_double(st + STEP_ADV_DT));
+    gate[s] = done && tail_on[s] ? 1u : 0u;
+}
+
+template <int D, int P>
+__global__ void __launch_bounds__(kThreads) kb_update_position(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+                                                               const uint32_t *__restrict__ map,
+                                                               const uint32_t *__restrict__ gate)
+{
+    const uint32_t m = map[blockIdx.x];
+    const uint32_t sc = map_scene(m);
+    if (!gate[sc])
+        return;
+    const SceneArgs<typename Prec<P>::T> &sa = tab[sc];
+    position_body<D, typename Prec<P>::T>(sa.pa, sa.Vol, sa.posvol, sa.pos_f32, map_block(m) * kThreads + threadIdx.x);
 }
 
 } // namespace
@@ -1672,7 +1882,7 @@ static void attach_relations(FluidArgs<T> &a, sphx_relation *inner, sphx_relatio
         a.chunk = inner->chunk_offset;
         a.count = inner->neighbor_size;
         a.slice = inner->slice;
-        if (inner->staged)
+        if (inner->staged && !inner->stage_stale)
         {
             a.stage_count = inner->stage_count;
             a.stage_keys = inner->stage_keys;
@@ -2342,6 +2552,142 @@ sphx_status substep_begin(sphx_context *ctx, double acoustic_cfl, double h)
     hipLaunchKernelGGL(k_substep_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->scalars_d, acoustic_cfl, h);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
+}
+
+//=============================================================================
+// batched launches (host side)
+//=============================================================================
+size_t batch_scene_args_bytes(int P) { return P == P_MIXED ? sizeof(SceneArgs<float>) : sizeof(SceneArgs<double>); }
+
+template <class T>
+static void fill_scene_args(void *dst, sphx_relation *inner, sphx_relation *const *w, int nw,
+                            const sphx_advection_config *cfg, const double *time_dev, bool staged)
+{
+    sphx_body *b = inner->source;
+    SceneArgs<T> a{};
+    a.f = fluid_args<T>(b);
+    a.f.step = b->ctx->scalars_d + SLOT_STEP;
+    a.f.lanes = b->ctx->scalars_d + SLOT_MAX;
+    a.f.riemann = cfg->riemann;
+    attach_relations(a.f, inner, w, nw);
+    if (!staged)
+    {
+        a.f.stage_count = nullptr;
+        a.f.stage_keys = nullptr;
+        a.f.local_slice = nullptr;
+        a.f.stage_cap = 0;
+    }
+    a.pa = pos_args(b);
+    a.Vol = (T *)b->Vol;
+    a.posvol = (Rec4<T> *)b->posvol;
+    a.pos_f32 = b->pos_f32;
+    a.scalars = b->ctx->scalars_d;
+    a.time = time_dev;
+    a.u_ref = cfg->u_ref;
+    a.advection_cfl = cfg->advection_cfl;
+    a.acoustic_cfl = cfg->acoustic_cfl;
+    a.h = b->h;
+    a.flow = cfg->flow_type;
+    std::memcpy(dst, &a, sizeof(a));
+}
+
+void batch_scene_args(void *dst, sphx_relation *inner, sphx_relation *const *w, int nw,
+                      const sphx_advection_config *cfg, const double *time_dev, bool staged)
+{
+    if (inner->source->P == P_MIXED)
+        fill_scene_args<float>(dst, inner, w, nw, cfg, time_dev, staged);
+    else
+        fill_scene_args<double>(dst, inner, w, nw, cfg, time_dev, staged);
+}
+
+template <int D, int P, bool RIEMANN>
+struct BatchL
+{
+    using T = typename Prec<P>::T;
+    static const SceneArgs<T> *tab(const BatchFluid &b) { return (const SceneArgs<T> *)b.tab; }
+    // dynamic LDS of the staged neighbour passes (as the single-scene launchers)
+    static size_t lds_sum(const BatchFluid &b) { return (size_t)b.stage_cap * sizeof(Rec4<T>); }
+    static size_t lds_pass1(const BatchFluid &b) { return (size_t)b.stage_cap * (sizeof(Rec4<T>) + sizeof(T)); }
+    static size_t lds_pass2(const BatchFluid &b)
+    {
+        return b.stage_cap ? (size_t)kPass2Cap * (sizeof(Rec4<T>) + 3 * sizeof(T)) : 0;
+    }
+    static sphx_status head(const BatchFluid &b)
+    {
+        const dim3 g(b.blocks), blk(kThreads), gs(grid_for(b.n_scenes, 64)), bs(64);
+        allow_lds(kb_summation<D, P>, lds_sum(b));
+        hipLaunchKernelGGL((kb_summation<D, P>), g, blk, lds_sum(b), b.stream, tab(b), b.map);
+        hipLaunchKernelGGL((kb_regularize_setup<D, P>), g, blk, 0, b.stream, tab(b), b.map);
+        hipLaunchKernelGGL((kb_reset_lanes<T>), dim3(b.n_scenes), dim3(64), 0, b.stream, tab(b), b.n_scenes);
+        hipLaunchKernelGGL((kb_speed_max<D, P>), g, blk, 0, b.stream, tab(b), b.map);
+        hipLaunchKernelGGL((kb_step_begin<T>), gs, bs, 0, b.stream, tab(b), b.n_scenes);
+        hipLaunchKernelGGL((kb_signal_max<D, P>), g, blk, 0, b.stream, tab(b), b.map);
+        SPHX_HIP(hipGetLastError());
+        return SPHX_OK;
+    }
+    static sphx_status substep(const BatchFluid &b)
+    {
+        const dim3 g(b.blocks), blk(kThreads), gs(grid_for(b.n_scenes, 64)), bs(64);
+        allow_lds(kb_pass1<D, P, RIEMANN>, lds_pass1(b));
+        allow_lds(kb_pass2<D, P, RIEMANN>, lds_pass2(b));
+        hipLaunchKernelGGL((kb_substep_begin<T>), gs, bs, 0, b.stream, tab(b), b.n_scenes);
+        hipLaunchKernelGGL((kb_init_1st<D, P>), g, blk, 0, b.stream, tab(b), b.map);
+        hipLaunchKernelGGL((kb_pass1<D, P, RIEMANN>), g, blk, lds_pass1(b), b.stream, tab(b), b.map);
+        hipLaunchKernelGGL((kb_pass2<D, P, RIEMANN>), g, blk, lds_pass2(b), b.stream, tab(b), b.map);
+        SPHX_HIP(hipGetLastError());
+        return SPHX_OK;
+    }
+    static sphx_status gate(const BatchFluid &b)
+    {
+        hipLaunchKernelGGL((kb_gate<T>), dim3(grid_for(b.n_scenes, 64)), dim3(64), 0, b.stream, tab(b), b.n_scenes,
+                           b.tail_on, b.gate, b.state);
+        SPHX_HIP(hipGetLastError());
+        return SPHX_OK;
+    }
+    static sphx_status position(const BatchFluid &b)
+    {
+        hipLaunchKernelGGL((kb_update_position<D, P>), dim3(b.blocks), dim3(kThreads), 0, b.stream, tab(b), b.map,
+                           b.gate);
+        SPHX_HIP(hipGetLastError());
+        return SPHX_OK;
+    }
+};
+
+template <template <int, int, bool> class L>
+struct BatchDispatch
+{
+    template <class F>
+    static sphx_status run(const BatchFluid &b, F &&f)
+    {
+        const bool rs = b.riemann != SPHX_RIEMANN_NONE;
+#define SPHX_BATCH_CASE(DD, PP)                                                                                   \
+    if (b.D == DD && b.P == PP)                                                                                  \
+        return rs ? f(L<DD, PP, true>{}) : f(L<DD, PP, false>{});
+        SPHX_BATCH_CASE(2, P_FP64)
+        SPHX_BATCH_CASE(3, P_FP64)
+        SPHX_BATCH_CASE(2, P_MIXED)
+        SPHX_BATCH_CASE(3, P_MIXED)
+#undef SPHX_BATCH_CASE
+        set_error("batched launch: unsupported dimension or precision");
+        return SPHX_E_ARG;
+    }
+};
+
+sphx_status batch_step_head(const BatchFluid &b)
+{
+    return BatchDispatch<BatchL>::run(b, [&](auto l) { return decltype(l)::head(b); });
+}
+sphx_status batch_substep(const BatchFluid &b)
+{
+    return BatchDispatch<BatchL>::run(b, [&](auto l) { return decltype(l)::substep(b); });
+}
+sphx_status batch_step_gate(const BatchFluid &b)
+{
+    return BatchDispatch<BatchL>::run(b, [&](auto l) { return decltype(l)::gate(b); });
+}
+sphx_status batch_position(const BatchFluid &b)
+{
+    return BatchDispatch<BatchL>::run(b, [&](auto l) { return decltype(l)::position(b); });
 }
 
 } // namespace sphx

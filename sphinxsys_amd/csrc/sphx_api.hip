@@ -417,7 +417,7 @@ sphx_status transfer(sphx_body *b, int field, void *host, bool write)
 
 using namespace sphx;
 
-namespace
+namespace sphx
 {
 bool relation_ready(sphx_relation *r)
 {
@@ -444,7 +444,85 @@ sphx_status check_fluid_relations(sphx_relation *inner, sphx_relation *const *wa
     }
     return SPHX_OK;
 }
-} // namespace
+} // namespace sphx
+
+namespace sphx
+{
+// dambreak.cpp:196-207: while (relaxation < advection dt) { dt = acoustic dt; 1st
+// half; 2nd half; relaxation += dt }.  Each sub-step's dt, the loop condition
+// and the time are evaluated on the device (k_substep_begin); launches past
+// the end of the loop return at once.  The first batch has `batch` sub-steps;
+// when it was too short the loop goes on one sub-step at a time.  On return
+// the step state is in the context's pinned mirror (SLOT_STEP).
+sphx_status advection_substeps(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                               const sphx_advection_config *cfg, uint32_t batch)
+{
+    sphx_body *b = inner->source;
+    sphx_context *ctx = b->ctx;
+    const bool corrected = cfg->correction == SPHX_CORRECTION_LINEAR;
+    struct DeviceStep
+    {
+        sphx_context *c;
+        explicit DeviceStep(sphx_context *c) : c(c) { c->device_step = true; }
+        ~DeviceStep() { c->device_step = false; }
+    } device_step(ctx);
+    const unsigned long long *st = ctx->scalars_h + SLOT_STEP;
+    sphx_status s;
+    while (true)
+    {
+        for (uint32_t k = 0; k < batch; ++k)
+        {
+            if ((s = substep_begin(ctx, cfg->acoustic_cfl, b->h)))
+                return s;
+            // k_substep_begin read the signal max and reset the lanes for pass 2
+            if (corrected)
+                s = acoustic_1st_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, true);
+            else
+                s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, 0.0, true);
+            if (s)
+                return s;
+            if (corrected)
+                s = acoustic_2nd_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, false, true);
+            else
+                s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, 0.0, false, true);
+            if (s)
+                return s;
+        }
+        if ((s = read_scalars(ctx, SLOT_STEP, STEP_SLOTS)))
+            return s;
+        if (st[STEP_ERROR])
+        {
+            set_error("non-finite acoustic time step: the fluid state diverged");
+            return SPHX_E_NONFINITE;
+        }
+        double relaxation, adv_dt;
+        std::memcpy(&relaxation, st + STEP_RELAXATION, sizeof(double));
+        std::memcpy(&adv_dt, st + STEP_ADV_DT, sizeof(double));
+        if (!(relaxation < adv_dt))
+            return SPHX_OK;
+        batch = 1;
+    }
+}
+
+// the end of an advection step: position update, ParticleSortCK when due, the
+// fluid cell linked list and the relations
+sphx_status advection_tail(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                           sphx_cell_linked_list *cll, bool sort_now)
+{
+    sphx_body *b = inner->source;
+    sphx_status s;
+    if ((s = sphx_update_particle_position(b)))
+        return s;
+    if (sort_now && (s = sphx_particle_sort(b, cll)))
+        return s;
+    if ((s = sphx_update_cell_linked_list(cll)) || (s = sphx_update_relation(inner)))
+        return s;
+    for (int k = 0; k < n_wall; ++k)
+        if ((s = sphx_update_relation(walls[k])))
+            return s;
+    return SPHX_OK;
+}
+} // namespace sphx
 
 extern "C" {
 
@@ -1047,6 +1125,7 @@ sphx_status sphx_update_relation(sphx_relation *r)
     r->slice_total = nchunks * kmax * 64;
     if ((s = relation_stage(r)))
         return s;
+    r->stage_stale = false;
     r->csr_valid = false; // the CSR view (offsets and indices) is built when read
     r->built = true;
     r->builds++;
@@ -1447,65 +1526,14 @@ sphx_status sphx_advection_step(sphx_relation *inner, sphx_relation *const *wall
     // left the max lanes reset)
     if ((s = acoustic_signal_max(b, ctx->scalars_d + SLOT_MAX)))
         return s;
-    struct DeviceStep
-    {
-        sphx_context *c;
-        explicit DeviceStep(sphx_context *c) : c(c) { c->device_step = true; }
-        ~DeviceStep() { c->device_step = false; }
-    } device_step(ctx);
-    // dambreak.cpp:196-207: while (relaxation < advection dt) { dt = acoustic dt; 1st
-    // half; 2nd half; relaxation += dt }.  Each sub-step's dt, the loop condition
-    // and the time are evaluated on the device (k_substep_begin); launches past
-    // the end of the loop return at once.  A batch is as long as the previous
-    // step's loop; when it was too short the loop goes on one sub-step at a time.
+    if ((s = advection_substeps(inner, walls, n_wall, cfg, ctx->last_substeps ? ctx->last_substeps : 1u)))
+        return s;
     const unsigned long long *st = ctx->scalars_h + SLOT_STEP;
-    uint32_t batch = ctx->last_substeps ? ctx->last_substeps : 1u;
-    while (true)
-    {
-        for (uint32_t k = 0; k < batch; ++k)
-        {
-            if ((s = substep_begin(ctx, cfg->acoustic_cfl, b->h)))
-                return s;
-            // k_substep_begin read the signal max and reset the lanes for pass 2
-            if (corrected)
-                s = acoustic_1st_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, true);
-            else
-                s = acoustic_1st_half(inner, walls, n_wall, cfg->riemann, 0.0, true);
-            if (s)
-                return s;
-            if (corrected)
-                s = acoustic_2nd_half_corrected(inner, walls, n_wall, cfg->riemann, 0.0, false, true);
-            else
-                s = acoustic_2nd_half(inner, walls, n_wall, cfg->riemann, 0.0, false, true);
-            if (s)
-                return s;
-        }
-        if ((s = read_scalars(ctx, SLOT_STEP, STEP_SLOTS)))
-            return s;
-        if (st[STEP_ERROR])
-        {
-            set_error("non-finite acoustic time step: the fluid state diverged");
-            return SPHX_E_NONFINITE;
-        }
-        double relaxation, adv_dt;
-        std::memcpy(&relaxation, st + STEP_RELAXATION, sizeof(double));
-        std::memcpy(&adv_dt, st + STEP_ADV_DT, sizeof(double));
-        if (!(relaxation < adv_dt))
-            break;
-        batch = 1;
-    }
     const uint32_t count = (uint32_t)st[STEP_COUNT];
     std::memcpy(time, st + STEP_TIME, sizeof(double));
     ctx->last_substeps = count;
-    if ((s = sphx_update_particle_position(b)))
+    if ((s = advection_tail(inner, walls, n_wall, cll, cfg->sort_now != 0)))
         return s;
-    if (cfg->sort_now && (s = sphx_particle_sort(b, cll)))
-        return s;
-    if ((s = sphx_update_cell_linked_list(cll)) || (s = sphx_update_relation(inner)))
-        return s;
-    for (int k = 0; k < n_wall; ++k)
-        if ((s = sphx_update_relation(walls[k])))
-            return s;
     if (steps)
         *steps = count;
     return SPHX_OK;

@@ -179,6 +179,7 @@ struct sphx_relation
     // stage_count[b] of them, or kStageOverflow) and the neighbour entries as
     // 16-bit ranks into that set (local_slice, same slice layout)
     bool staged = false;
+    bool stage_stale = false;             // lists rebuilt without their staging sets (the passes gather directly)
     uint32_t stage_cap = 0;
     uint32_t *stage_count = nullptr;
     uint32_t *stage_keys = nullptr;
@@ -269,6 +270,87 @@ sphx_status mechanical_energy(sphx_body *b, const double *g, double *out_device)
 sphx_status interpolate_contact(sphx_relation *rel, int field, int component);
 sphx_status fill_u64(sphx_context *ctx, unsigned long long *p, size_t n, unsigned long long v);
 sphx_status build_kernel_poly(sphx_relation *rel);
+
+// ---- batched launches over the scenes of a batch (fluid_kernels.hip) ----
+// A batch's scene table holds one record per scene (batch_scene_args_bytes
+// each, written by batch_scene_args); map[b] = scene << 16 | block of that
+// scene over the 256-particle blocks of every scene.
+struct BatchFluid
+{
+    int D = 2, P = 0, riemann = 0;
+    hipStream_t stream = nullptr;
+    const void *tab = nullptr;
+    uint32_t n_scenes = 0;
+    const uint32_t *map = nullptr;
+    uint32_t blocks = 0;
+    uint32_t stage_cap = 0;            // staging capacity of the staged scenes (0: none staged)
+    const uint32_t *tail_on = nullptr; // per scene: the device runs the step's tail
+    uint32_t *gate = nullptr;          // per scene: sub-steps done and tail on (written by the gate)
+    unsigned long long *state = nullptr; // per scene: STEP_SLOTS copies of the step state
+};
+size_t batch_scene_args_bytes(int P);
+// time_dev: the scene's physical time at the start of the step (device);
+// staged false: the passes gather from global memory (the staging sets unused)
+void batch_scene_args(void *dst, sphx_relation *inner, sphx_relation *const *w, int nw,
+                      const sphx_advection_config *cfg, const double *time_dev, bool staged);
+// compression summation, regularization + setup, advection dt, first acoustic signal max
+sphx_status batch_step_head(const BatchFluid &b);
+// one acoustic sub-step: its dt and loop flag, 1st half (with the 2nd half's initialize), 2nd half + CFL
+sphx_status batch_substep(const BatchFluid &b);
+sphx_status batch_step_gate(const BatchFluid &b);
+sphx_status batch_position(const BatchFluid &b);
+
+// ---- batched configuration update (config_kernels.hip) ----
+// Per scene: cell list of the fluid (keys, counts, scan, scatter, in-cell
+// order); per relation: the one-pass tiled build and, for staged inner
+// relations, the staging sets.  Gated per scene like the fluid tail.
+struct BatchConfig
+{
+    int D = 2;
+    hipStream_t stream = nullptr;
+    const uint32_t *gate = nullptr;
+    // cell lists: table, 256-particle map (as BatchFluid), 256-cell map over
+    // ncells + 1, scan-tile map over ncells + 1
+    const void *cll = nullptr;
+    uint32_t n_scenes = 0;
+    const uint32_t *map_particles = nullptr;
+    uint32_t blocks_particles = 0;
+    const uint32_t *map_cells = nullptr;
+    uint32_t blocks_cells = 0;
+    const uint32_t *map_tiles = nullptr;
+    uint32_t blocks_tiles = 0;
+    // relations: build table, 64-particle chunk map (relation << 16 | chunk), scene of each relation
+    const void *build = nullptr;
+    uint32_t n_rel = 0;
+    const uint32_t *map_chunks = nullptr;
+    uint32_t blocks_chunks = 0;
+    const uint32_t *rel_scene = nullptr;
+    unsigned long long *needed = nullptr; // per relation, zeroed here before the build
+    int tile_waves = 1;                   // waves per workgroup of the tiled build (1, 2 or 4)
+    // staging: table per scene (n_src 0: not staged), 256-particle map
+    const void *stage = nullptr;
+};
+size_t batch_cll_args_bytes();
+// cursor: ncells words, tile_sums: scan tiles of ncells + 1 (device, owned by the batch)
+void batch_cll_args(void *dst, sphx_cell_linked_list *cll, uint32_t *cursor, uint32_t *tile_sums);
+uint32_t batch_cll_tiles(const sphx_cell_linked_list *cll);
+size_t batch_build_args_bytes();
+void batch_build_args(void *dst, sphx_relation *rel, unsigned long long *needed);
+size_t batch_stage_args_bytes();
+void batch_stage_args(void *dst, sphx_relation *rel); // rel null or not staged: an empty record
+sphx_status batch_cell_linked_list(const BatchConfig &b);
+sphx_status batch_relations(const BatchConfig &b);
+
+// ---- pieces of the advection step driver (sphx_api.hip) ----
+bool relation_ready(sphx_relation *r);
+sphx_status check_fluid_relations(sphx_relation *inner, sphx_relation *const *walls, int n_wall, bool need_built);
+// the device-side sub-step loop from the current step state until it ends;
+// the state is then in the context's pinned mirror
+sphx_status advection_substeps(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                               const sphx_advection_config *cfg, uint32_t batch);
+// position update, [ParticleSortCK], cell linked list, relations
+sphx_status advection_tail(sphx_relation *inner, sphx_relation *const *walls, int n_wall,
+                           sphx_cell_linked_list *cll, bool sort_now);
 
 // ---- scalar read-back helper (sphx_api.hip) ----
 sphx_status read_scalars(sphx_context *ctx, int first, int count);

@@ -1,14 +1,14 @@
 """Batched and served dam-break scenes on one GPU.
 
-Many independent scenes of uneven size advance together: each admitted scene
-owns a context (its own HIP stream and device state, resident until the scene
-finishes) and a host thread that drives its dual-time loop through the C ABI.
-The C calls release the interpreter lock, so the scenes' kernels are in
-flight on their streams at the same time and small scenes fill the GPU
-together.  A request loop admits scenes at their arrival time when a slot and
-enough of the HBM budget are free (continuous batching: requests that arrive
-while others run are admitted as slots and memory free up), and records each
-scene's time to solution.
+Many independent scenes of uneven size advance together.  `BatchServer`
+runs them on the batch path (sphx_batch_*): every stage of an advection step
+is one launch over all admitted scenes, the step is one graph launch, and a
+request loop admits arriving scenes when a slot and enough of the HBM budget
+are free (continuous batching: requests that arrive while others run join at
+the next step; finished scenes leave) and records each scene's time to
+solution.  `Server` is the stream-per-scene form: each admitted scene owns a
+context and a host thread that drives its own loop through the C ABI; with
+one slot it is the serial baseline (one plain run after another).
 
 A scene's result does not depend on what else runs: its state is private and
 every kernel is deterministic, so the same request alone and in a mix gives
@@ -194,6 +194,121 @@ class Server:
             th.join()
         if state["error"] is not None:
             raise state["error"]
+        return results, time.perf_counter() - t0
+
+
+class BatchServer:
+    """Continuous batching on the batch path: admitted scenes advance together,
+    one batched advection step (one graph launch) at a time; a request that
+    arrives while others run is admitted at the next step once a slot and
+    enough of the HBM budget are free, a scene leaves the batch when it
+    reaches its end, and each scene's time to solution is recorded.  A worker
+    thread builds and initialises arriving scenes (on their own contexts)
+    while the batch steps, so admission overlaps device work.  Every scene's
+    result is the one it has alone (the batch is bit-identical per scene)."""
+
+    def __init__(self, lib, max_active=256, hbm_budget=200 << 30, options=None):
+        self.lib, self.max_active, self.hbm_budget, self.options = lib, max_active, hbm_budget, options
+        self.tuning = None
+
+    def run(self, specs, keep_fields=False, realtime=True):
+        import queue
+        results = [SceneResult(s.ident, 0, arrival=s.arrival) for s in specs]
+        order = sorted(range(len(specs)), key=lambda k: specs[k].arrival)
+        ready = queue.Queue()
+        admit = threading.Semaphore(0)
+        state = {"bytes": 0, "error": None}
+        lock = threading.Lock()
+        t0 = time.perf_counter()
+
+        def builder():
+            # builds scenes in arrival order, each once admission allows it
+            for k in order:
+                spec = specs[k]
+                need = spec.footprint_bytes()
+                if need > self.hbm_budget:
+                    state["error"] = ValueError(f"scene {spec.ident} needs {need} B, more than the HBM budget")
+                    ready.put(None)
+                    return
+                if realtime:
+                    wait = spec.arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(wait)
+                else:
+                    results[k].arrival = time.perf_counter() - t0
+                admit.acquire()  # a slot
+                while True:
+                    with lock:
+                        if state["bytes"] + need <= self.hbm_budget:
+                            state["bytes"] += need
+                            break
+                    time.sleep(0.001)
+                try:
+                    sc = cases.DamBreak(self.lib, spec.geometry(), spec.precision)
+                    sc.initialize()
+                    sc.ctx.synchronize()
+                except Exception as e:  # reported by the stepping loop
+                    state["error"] = e
+                    ready.put(None)
+                    return
+                ready.put((k, sc, need))
+
+        for _ in range(self.max_active):
+            admit.release()
+        th = threading.Thread(target=builder, daemon=True)
+        th.start()
+        from . import ck
+        batch = ck.BatchAdvection(self.lib, [], options=self.options)
+        active = []  # (k, scene, bytes)
+        done = 0
+        try:
+            while done < len(specs):
+                # admit what is ready; wait for the next scene when nothing runs
+                while True:
+                    try:
+                        item = ready.get(block=not active, timeout=None if not active else 0)
+                    except queue.Empty:
+                        break
+                    if item is None:
+                        raise state["error"]
+                    k, sc, need = item
+                    results[k].start = time.perf_counter() - t0
+                    results[k].n_fluid = sc.fluid.n
+                    batch.add(sc.advection, 0.0)
+                    active.append((k, sc, need))
+                # ParticleSortCK where each scene's own loop sorts (cases.DamBreak.advection_step)
+                st = batch.exec(sort_now=[1 if (sc.iterations + 1) % 100 == 0 else 0 for _, sc, _ in active])
+                if (st != 0).any():
+                    bad = [active[i][0] for i in np.flatnonzero(st)]
+                    raise capi.SphxError(f"scenes {bad} diverged")
+                for i, (k, sc, _) in enumerate(active):
+                    sc.time = float(batch.time[i])
+                    sc.iterations += 1
+                    sc.acoustic_steps += int(batch.steps[i])
+                finished = [i for i, (k, sc, _) in enumerate(active)
+                            if (specs[k].steps and sc.iterations >= specs[k].steps)
+                            or (specs[k].t_end and sc.time >= specs[k].t_end)]
+                for i in reversed(finished):
+                    k, sc, need = active.pop(i)
+                    batch.remove(i)
+                    res = results[k]
+                    res.finish = time.perf_counter() - t0
+                    res.acoustic_steps = sc.acoustic_steps
+                    res.advection_steps = sc.iterations
+                    res.physical_time = sc.time
+                    if keep_fields:
+                        res.fields = {f: sc.fluid.read(f) for f in ("Position", "Velocity", "Compression")}
+                    sc.close()
+                    with lock:
+                        state["bytes"] -= need
+                    admit.release()
+                    done += 1
+            self.tuning = batch.tuning()
+        finally:
+            batch.close()
+            for _, sc, _ in active:
+                sc.close()
+        th.join(timeout=60)
         return results, time.perf_counter() - t0
 
 
