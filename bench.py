@@ -109,7 +109,8 @@ def stage_breakdown(sc, steps):
     sc.ctx.synchronize()
     prof = {k: sc.ctx.profile_read(k) for k in (
         "acoustic_pass1", "acoustic_pass2", "acoustic_init", "compression_summation",
-        "relation_build", "relation_scan", "relation_stage", "cell_linked_list", "particle_sort",
+        "relation_build", "relation_build_contact", "relation_scan", "relation_stage", "cell_linked_list",
+        "particle_sort",
         "time_step_reduce", "state_update")}
     sc.ctx.profile(False)
     return prof
@@ -449,23 +450,24 @@ def main():
             "mean_neighbors": k_bar,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "valu" if valu_frac > hbm_frac else "hbm",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": hbm_frac,
-            "traffic": profiled_traffic(dom),
-            "traffic_source": "profiles/dambreak3d_8M_mixed.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)",
-            "bytes_per_particle": dom_b,
-            "avg_launch_ms": avg_ms,
-            "valu": {"achieved": flops, "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": valu_frac,
-                     "flop_per_particle": dom_f,
-                     "note": "algorithmic FP32 flops of the pair loops (FMA = 2) at the measured mean "
-                             "neighbour counts; the published peak counts packed FP32, which this build "
-                             "does not issue, so the issue-limited ceiling is half of it"},
-        },
+        # the roof that binds the dominant kernel (the larger fraction) gives the
+        # top-level figures; the other roof is reported beside it
+        "roofline": dict(
+            ({"bound": "valu", "achieved": flops, "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+              "frac": valu_frac} if valu_frac > hbm_frac else
+             {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac}),
+            kernel=dom,
+            traffic=profiled_traffic(dom),
+            traffic_source="profiles/dambreak3d_8M_mixed.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)",
+            avg_launch_ms=avg_ms,
+            hbm={"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac,
+                 "bytes_per_particle": dom_b},
+            valu={"achieved": flops, "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": valu_frac,
+                  "flop_per_particle": dom_f,
+                  "note": "algorithmic FP32 flops of the pair loops (FMA = 2) at the measured mean "
+                          "neighbour counts against the published vector FP32 peak, which counts packed "
+                          "FP32; this build issues no packed FP32, so its issue-limited ceiling is half "
+                          "of that peak"}),
         "step_roofline": {
             "hbm_frac": STEP_BYTES["mixed"] * value / (HBM_PEAK_GBS * 1e9),
             "valu_frac": STEP_FLOP_PER_PAIR * k_bar * value / FP32_VALU_PEAK,

@@ -123,8 +123,9 @@ const char *sphx_last_error(void);
 /* ---- context ------------------------------------------------------------
  * The *_destroy calls wait for their own context's stream only; the device
  * memory and streams they release stay with the library for the next objects
- * of the process (the memory goes back to HIP when an allocation of the
- * library runs out of device memory). */
+ * of the process (at most 16 GiB of memory per device; beyond that a released
+ * block goes back to HIP at once, and all of it does when an allocation of
+ * the library runs out of device memory). */
 sphx_status sphx_context_create(int device, sphx_context **out);
 sphx_status sphx_context_destroy(sphx_context *ctx);
 sphx_status sphx_context_synchronize(sphx_context *ctx);
@@ -161,12 +162,12 @@ sphx_status sphx_cell_linked_list_mesh(const sphx_cell_linked_list *cll, int *al
 sphx_status sphx_update_cell_linked_list(sphx_cell_linked_list *cll);
 /* No reference counterpart: sphx_update_cell_linked_list for callers that
  * cannot allocate device memory (tight serving loops, harnesses that forbid
- * hipMalloc).  Same result; its workspace is a fixed device array of the
- * library (about 32 MiB per device, held until the process ends) instead of
- * the context's grow-only scratch, and it records no profiling events.  At
- * most 1,048,576 particles and 4,194,304 cells (SPHX_E_CAPACITY beyond).
- * Every call on one device shares that workspace: calls that overlap on one
- * device (contexts driven from several host threads) race. */
+ * hipMalloc).  Same result; its workspace (a cursor per cell and the scan's
+ * tile sums) is a fixed device array of the library (16 MiB per device, held
+ * until the process ends) instead of the context's grow-only scratch, and it
+ * records no profiling events.  At most 4,194,304 cells (SPHX_E_CAPACITY
+ * beyond).  Every call on one device shares that workspace: calls that
+ * overlap on one device (contexts driven from several host threads) race. */
 sphx_status sphx_update_cell_linked_list_noalloc(sphx_cell_linked_list *cll);
 /* cell_offset[n_cells+1], particle_index[n_particles] (per-cell index order
  * is ascending particle index; the reference fills it with relaxed atomics). */
@@ -391,7 +392,8 @@ sphx_status sphx_batch_tuning(const sphx_batch *batch, char *json, size_t capaci
  * launch of the named stage (no reference counterpart; the reference times
  * whole phases with TickCount, dambreak_sycl.cpp:175-259).  Stage names:
  * "acoustic_pass1", "acoustic_pass2", "acoustic_init", "compression_summation",
- * "relation_build" (tiled list build), "relation_scan" (per-lane scan build,
+ * "relation_build" (tiled list build of inner relations), "relation_build_contact"
+ * (tiled list build of contact relations), "relation_scan" (per-lane scan build,
  * used when a list outgrows the LDS tile), "relation_stage" (LDS staging sets
  * of the FP32-mixed inner relation), "cell_linked_list", "particle_sort",
  * "time_step_reduce", "state_update", "linear_correction" (correction matrix and

@@ -35,7 +35,6 @@ struct Shape
     bool operator==(const Shape &o) const { return tile_waves == o.tile_waves && stage_min == o.stage_min; }
 };
 
-constexpr uint32_t kNoStaging = 0xffffffffu;
 constexpr uint32_t kMapLimit = 1u << 16; // scenes, relations and blocks per scene in a block map entry
 } // namespace
 
@@ -485,17 +484,17 @@ void tuning_candidates(sphx_batch *b)
     b->tuned = !b->opt.autotune;
     if (!b->opt.autotune)
         return;
-    // which scenes are staged (all, from 4096 and from 16384 particles, none),
-    // then the tiled build's waves per workgroup for the best of those
-    for (uint32_t smin : {0u, 4096u, 16384u, kNoStaging})
+    // which scenes are staged (all, from 4096, 16384 or 65536 fluid particles;
+    // larger scenes always are, as in the single-scene path, whose 3D passes
+    // depend on it), then the tiled build's waves per workgroup for the best
+    for (uint32_t smin : {0u, 4096u, 16384u, 65536u})
         b->candidates.push_back(Shape{1, smin});
 }
 
 std::string shape_json(const Shape &s)
 {
     char buf[96];
-    std::snprintf(buf, sizeof(buf), "{\"tile_waves\": %d, \"stage_min\": %lld}", s.tile_waves,
-                  s.stage_min == kNoStaging ? -1ll : (long long)s.stage_min);
+    std::snprintf(buf, sizeof(buf), "{\"tile_waves\": %d, \"stage_min\": %u}", s.tile_waves, s.stage_min);
     return buf;
 }
 } // namespace

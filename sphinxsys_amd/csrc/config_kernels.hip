@@ -331,120 +331,6 @@ __global__ void __launch_bounds__(kThreads) k_cell_keys(PosPtrs pos, uint32_t n,
     cell_keys_body<D>(pos, n, m, keys, counts, blockIdx.x);
 }
 
-// runs of equal cell keys in particle order: flag[i] = 1 where a run starts
-__global__ void k_run_flags(const uint32_t *keys, uint32_t n, uint32_t *flag)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        flag[i] = (i == 0u || keys[i] != keys[i - 1u]) ? 1u : 0u;
-}
-
-// run r (ex[i] at its first particle i): its key, its number and its start
-__global__ void k_run_emit(const uint32_t *keys, const uint32_t *ex, uint32_t n, uint32_t *run_key,
-                           uint32_t *run_idx, uint32_t *run_start)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (i == 0u || keys[i] != keys[i - 1u]))
-    {
-        const uint32_t r = ex[i];
-        run_key[r] = keys[i];
-        run_idx[r] = r;
-        run_start[r] = i;
-    }
-}
-
-// length of the run at sorted position s
-__global__ void k_run_len(const uint32_t *run_idx, const uint32_t *run_start, uint32_t nruns, uint32_t n,
-                          uint32_t *len)
-{
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nruns)
-    {
-        const uint32_t r = run_idx[s];
-        len[s] = (r + 1u < nruns ? run_start[r + 1u] : n) - run_start[r];
-    }
-}
-
-// first output slot of each run, by run number
-__global__ void k_run_off(const uint32_t *run_idx, const uint32_t *soff, uint32_t nruns, uint32_t *run_off)
-{
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nruns)
-        run_off[run_idx[s]] = soff[s];
-}
-
-// particle i goes to its run's first slot plus its place in the run
-__global__ void k_run_place(const uint32_t *keys, const uint32_t *ex, const uint32_t *run_start,
-                            const uint32_t *run_off, uint32_t n, uint32_t *particle_index)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-    {
-        const uint32_t head = (i == 0u || keys[i] != keys[i - 1u]) ? 1u : 0u;
-        const uint32_t r = ex[i] + head - 1u;
-        particle_index[run_off[r] + (i - run_start[r])] = i;
-    }
-}
-
-// the runs of a small set (at most kSmallRuns) in one workgroup: sorted by
-// (cell, run number) with a bitonic network in LDS, which is the order of the
-// stable sort, then each run's output offset from the exclusive sum of the run
-// lengths in that order
-constexpr uint32_t kSmallRuns = 4096;
-constexpr uint32_t kSmallThreads = 1024;
-__global__ void __launch_bounds__(kSmallThreads) k_runs_small(const uint32_t *run_key, const uint32_t *run_start,
-                                                              uint32_t nruns, uint32_t n, uint32_t *run_off)
-{
-    __shared__ unsigned long long v[kSmallRuns];
-    __shared__ uint32_t wave_tot[kSmallThreads / 64];
-    const uint32_t t = threadIdx.x;
-    uint32_t p = 1;
-    while (p < nruns)
-        p <<= 1;
-    for (uint32_t q = t; q < p; q += kSmallThreads)
-        v[q] = q < nruns ? ((unsigned long long)run_key[q] << 32) | q : ~0ull;
-    __syncthreads();
-    for (uint32_t k = 2; k <= p; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1)
-        {
-            for (uint32_t a = t; a < p; a += kSmallThreads)
-            {
-                const uint32_t b = a ^ j;
-                if (b > a)
-                {
-                    const unsigned long long x = v[a], y = v[b];
-                    if ((x > y) == ((a & k) == 0))
-                    {
-                        v[a] = y;
-                        v[b] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    // thread t owns sorted positions [t * per, (t + 1) * per)
-    const uint32_t per = (nruns + kSmallThreads - 1) / kSmallThreads;
-    const uint32_t lo = min(nruns, t * per), hi = min(nruns, lo + per);
-    auto length = [&](uint32_t r) { return (r + 1u < nruns ? run_start[r + 1u] : n) - run_start[r]; };
-    uint32_t sum = 0;
-    for (uint32_t q = lo; q < hi; ++q)
-        sum += length((uint32_t)(v[q] & 0xffffffffu));
-    const uint32_t inc = wave_inclusive_scan(sum);
-    const uint32_t lane = t & 63u, wave = t >> 6;
-    if (lane == 63u)
-        wave_tot[wave] = inc;
-    __syncthreads();
-    uint32_t base = inc - sum;
-    for (uint32_t w = 0; w < wave; ++w)
-        base += wave_tot[w];
-    for (uint32_t q = lo; q < hi; ++q)
-    {
-        const uint32_t r = (uint32_t)(v[q] & 0xffffffffu);
-        run_off[r] = base;
-        base += length(r);
-    }
-}
-
 template <int D>
 __global__ void __launch_bounds__(kThreads) k_morton_keys(PosPtrs pos, uint32_t n, MeshParams m, uint32_t *keys,
                                                           uint32_t *vals)
@@ -1288,6 +1174,48 @@ struct CllScene
 __device__ __forceinline__ uint32_t bmap_owner(uint32_t m) { return m >> 16; }
 __device__ __forceinline__ uint32_t bmap_block(uint32_t m) { return m & 0xffffu; }
 
+// counts and cursors of the cells to zero
+__device__ __forceinline__ void cll_clear_body(const CllScene &a, uint32_t c)
+{
+    if (c <= a.ncells)
+        a.cell_offset[c] = 0u;
+    if (c < a.ncells)
+        a.cursor[c] = 0u;
+}
+// particle i at its cell's offset plus a slot taken atomically
+__device__ __forceinline__ void cll_scatter_body(const CllScene &a, uint32_t i)
+{
+    if (i >= a.n)
+        return;
+    const uint32_t key = a.keys[i];
+    a.particle_index[a.cell_offset[key] + atomicAdd(&a.cursor[key], 1u)] = i;
+}
+// the indices of cell c in ascending order (cells hold a few particles)
+__device__ __forceinline__ void cll_order_body(const CllScene &a, uint32_t c)
+{
+    if (c >= a.ncells)
+        return;
+    const uint32_t lo = a.cell_offset[c], hi = a.cell_offset[c + 1];
+    uint32_t *idx = a.particle_index;
+    for (uint32_t q = lo + 1; q < hi; ++q)
+    {
+        const uint32_t v = idx[q];
+        uint32_t r = q;
+        while (r > lo && idx[r - 1] > v)
+        {
+            idx[r] = idx[r - 1];
+            --r;
+        }
+        idx[r] = v;
+    }
+}
+__global__ void __launch_bounds__(kThreads) k_cll_clear(CllScene a) { cll_clear_body(a, blockIdx.x * kThreads + threadIdx.x); }
+__global__ void __launch_bounds__(kThreads) k_cll_scatter(CllScene a)
+{
+    cll_scatter_body(a, blockIdx.x * kThreads + threadIdx.x);
+}
+__global__ void __launch_bounds__(kThreads) k_cll_order(CllScene a) { cll_order_body(a, blockIdx.x * kThreads + threadIdx.x); }
+
 __global__ void __launch_bounds__(kThreads) kb_cll_clear(const CllScene *__restrict__ tab,
                                                          const uint32_t *__restrict__ map,
                                                          const uint32_t *__restrict__ gate)
@@ -1295,12 +1223,7 @@ __global__ void __launch_bounds__(kThreads) kb_cll_clear(const CllScene *__restr
     const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
     if (!gate[sc])
         return;
-    const CllScene &a = tab[sc];
-    const uint32_t c = bmap_block(m) * kThreads + threadIdx.x;
-    if (c <= a.ncells)
-        a.cell_offset[c] = 0u;
-    if (c < a.ncells)
-        a.cursor[c] = 0u;
+    cll_clear_body(tab[sc], bmap_block(m) * kThreads + threadIdx.x);
 }
 
 template <int D>
@@ -1354,15 +1277,9 @@ __global__ void __launch_bounds__(kThreads) kb_cll_scatter(const CllScene *__res
     const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
     if (!gate[sc])
         return;
-    const CllScene &a = tab[sc];
-    const uint32_t i = bmap_block(m) * kThreads + threadIdx.x;
-    if (i >= a.n)
-        return;
-    const uint32_t key = a.keys[i];
-    a.particle_index[a.cell_offset[key] + atomicAdd(&a.cursor[key], 1u)] = i;
+    cll_scatter_body(tab[sc], bmap_block(m) * kThreads + threadIdx.x);
 }
 
-// the indices of each cell in ascending order (cells hold a few particles)
 __global__ void __launch_bounds__(kThreads) kb_cll_order(const CllScene *__restrict__ tab,
                                                          const uint32_t *__restrict__ map,
                                                          const uint32_t *__restrict__ gate)
@@ -1370,23 +1287,7 @@ __global__ void __launch_bounds__(kThreads) kb_cll_order(const CllScene *__restr
     const uint32_t m = map[blockIdx.x], sc = bmap_owner(m);
     if (!gate[sc])
         return;
-    const CllScene &a = tab[sc];
-    const uint32_t c = bmap_block(m) * kThreads + threadIdx.x;
-    if (c >= a.ncells)
-        return;
-    const uint32_t lo = a.cell_offset[c], hi = a.cell_offset[c + 1];
-    uint32_t *idx = a.particle_index;
-    for (uint32_t q = lo + 1; q < hi; ++q)
-    {
-        const uint32_t v = idx[q];
-        uint32_t r = q;
-        while (r > lo && idx[r - 1] > v)
-        {
-            idx[r] = idx[r - 1];
-            --r;
-        }
-        idx[r] = v;
-    }
+    cll_order_body(tab[sc], bmap_block(m) * kThreads + threadIdx.x);
 }
 
 __global__ void kb_zero_u64(unsigned long long *p, uint32_t n)
@@ -1492,38 +1393,25 @@ sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *
     return SPHX_OK;
 }
 
-sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits,
-                             const CllWork *work)
+sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits)
 {
     if (n <= 1 || bits <= 0)
         return SPHX_OK;
     sphx_status s;
     uint32_t ntiles = grid_for(n, kTile);
     size_t nhist = (size_t)kBuckets * ntiles;
-    uint32_t *hist, *k1, *v1, *sums = nullptr;
-    if (work)
-    {
-        hist = work->hist;
-        k1 = work->sort_keys;
-        v1 = work->sort_vals;
-        sums = work->scan_sums;
-    }
-    else
-    {
-        if ((s = ctx->sort_keys.reserve(n * sizeof(uint32_t))) || (s = ctx->sort_vals.reserve(n * sizeof(uint32_t))) ||
-            (s = ctx->sort_hist.reserve(nhist * sizeof(uint32_t))))
-            return s;
-        hist = (uint32_t *)ctx->sort_hist.ptr;
-        k1 = (uint32_t *)ctx->sort_keys.ptr;
-        v1 = (uint32_t *)ctx->sort_vals.ptr;
-    }
+    if ((s = ctx->sort_keys.reserve(n * sizeof(uint32_t))) || (s = ctx->sort_vals.reserve(n * sizeof(uint32_t))) ||
+        (s = ctx->sort_hist.reserve(nhist * sizeof(uint32_t))))
+        return s;
+    uint32_t *hist = (uint32_t *)ctx->sort_hist.ptr;
+    uint32_t *k1 = (uint32_t *)ctx->sort_keys.ptr, *v1 = (uint32_t *)ctx->sort_vals.ptr;
     uint32_t *k0 = keys, *v0 = vals;
     int passes = (bits + kRadixBits - 1) / kRadixBits;
     for (int p = 0; p < passes; ++p)
     {
         int shift = p * kRadixBits;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kThreads), 0, ctx->stream, k0, n, shift, hist, ntiles);
-        if ((s = exclusive_scan_u32(ctx, hist, hist, nhist, nullptr, sums)))
+        if ((s = exclusive_scan_u32(ctx, hist, hist, nhist, nullptr)))
             return s;
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kThreads), 0, ctx->stream, k0, v0, k1, v1, n, shift,
                            hist, ntiles);
@@ -1549,82 +1437,71 @@ static int bits_for(uint64_t max_value)
     return b;
 }
 
-// UpdateCellLinkedList<P,RealBody>::exec (update_cell_linked_list.hpp:75-106):
-// per-cell counts + exclusive scan give CellOffset; a stable sort of the cell
-// keys gives ParticleIndex in ascending particle order inside each cell.
+// UpdateCellLinkedList<P,RealBody>::exec (update_cell_linked_list.hpp:75-106)
+// as a counting sort: per-cell counts (one atomic per run of equal cells in a
+// wave) with the cell keys, an exclusive scan giving CellOffset, an atomic
+// scatter of the particles to their cells, then each cell's indices put in
+// ascending order, so that ParticleIndex is the canonical list whatever order
+// the atomics took.  No size is read back to the host.
 sphx_status cell_linked_list_update(sphx_cell_linked_list *cll, const CllWork *work)
 {
     sphx_body *b = cll->body;
     sphx_context *ctx = b->ctx;
     // the allocation-free form records no profiling events (their creation allocates)
     ProfileScope prof(ctx, work ? nullptr : "cell_linked_list");
-    uint32_t *sums = work ? work->scan_sums : nullptr;
     const size_t nc = cll->ncells;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nc + 1, 256)), dim3(256), 0, ctx->stream, cll->cell_offset,
-                       nc + 1, 0u);
     const uint32_t n = b->n;
+    CllScene a{};
+    a.pos = pos_ptrs(b);
+    a.n = n;
+    a.ncells = (uint32_t)nc;
+    a.m = cll->mesh;
+    a.keys = cll->keys;
+    a.cell_offset = cll->cell_offset;
+    a.particle_index = cll->particle_index;
+    uint32_t *sums = nullptr;
+    sphx_status s;
+    if (work)
+    {
+        a.cursor = work->cursor;
+        sums = work->scan_sums;
+    }
+    else
+    {
+        if ((s = ctx->cll_cursor.reserve(std::max<size_t>(nc, 1) * sizeof(uint32_t))))
+            return s;
+        a.cursor = (uint32_t *)ctx->cll_cursor.ptr;
+    }
+    const dim3 blk(kThreads);
+    hipLaunchKernelGGL(k_cll_clear, dim3(grid_for(nc + 1, kThreads)), blk, 0, ctx->stream, a);
     if (n > 0)
     {
         // counts accumulate in cell_offset, then scan in place
         if (b->D == 2)
-            hipLaunchKernelGGL(k_cell_keys<2>, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                               pos_ptrs(b), n, cll->mesh, cll->keys, cll->cell_offset);
+            hipLaunchKernelGGL(k_cell_keys<2>, dim3(grid_for(n, kThreads)), blk, 0, ctx->stream, a.pos, n, cll->mesh,
+                               cll->keys, cll->cell_offset);
         else
-            hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                               pos_ptrs(b), n, cll->mesh, cll->keys, cll->cell_offset);
+            hipLaunchKernelGGL(k_cell_keys<3>, dim3(grid_for(n, kThreads)), blk, 0, ctx->stream, a.pos, n, cll->mesh,
+                               cll->keys, cll->cell_offset);
+    }
+    SPHX_HIP(hipGetLastError());
+    if ((s = exclusive_scan_u32(ctx, cll->cell_offset, cll->cell_offset, nc + 1, nullptr, sums)))
+        return s;
+    if (n > 0)
+    {
+        hipLaunchKernelGGL(k_cll_scatter, dim3(grid_for(n, kThreads)), blk, 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_cll_order, dim3(grid_for(nc, kThreads)), blk, 0, ctx->stream, a);
         SPHX_HIP(hipGetLastError());
     }
-    sphx_status s = exclusive_scan_u32(ctx, cll->cell_offset, cll->cell_offset, nc + 1, nullptr, sums);
-    if (s || n == 0)
-        return s;
-    // particles by cell, ascending index within a cell: the runs of equal
-    // cells in particle order (as many as cells, plus two per particle that
-    // left its run since the last sort) are sorted stably by cell, and each
-    // particle is placed at its run's output offset plus its place in the run
-    uint32_t *ex = work ? work->runs : nullptr;
-    if (!ex)
-    {
-        if ((s = ctx->cll_runs.reserve((size_t)6 * n * sizeof(uint32_t))))
-            return s;
-        ex = (uint32_t *)ctx->cll_runs.ptr;
-    }
-    uint32_t *run_key = ex + n, *run_idx = run_key + n, *run_start = run_idx + n, *len = run_start + n,
-             *run_off = len + n;
-    const dim3 g(grid_for(n, 256)), blk(256);
-    hipLaunchKernelGGL(k_run_flags, g, blk, 0, ctx->stream, cll->keys, n, ex);
-    uint32_t nruns = 0;
-    if ((s = exclusive_scan_u32(ctx, ex, ex, n, &nruns, sums)))
-        return s;
-    hipLaunchKernelGGL(k_run_emit, g, blk, 0, ctx->stream, cll->keys, ex, n, run_key, run_idx, run_start);
-    SPHX_HIP(hipGetLastError());
-    if (nruns <= kSmallRuns)
-    {
-        // one launch instead of the radix passes (small scenes are bound by launch counts)
-        hipLaunchKernelGGL(k_runs_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, run_key, run_start, nruns, n,
-                           run_off);
-    }
-    else
-    {
-        if ((s = radix_sort_pairs(ctx, run_key, run_idx, nruns, cll->key_bits, work)))
-            return s;
-        const dim3 gr(grid_for(nruns, 256));
-        hipLaunchKernelGGL(k_run_len, gr, blk, 0, ctx->stream, run_idx, run_start, nruns, n, len);
-        if ((s = exclusive_scan_u32(ctx, len, len, nruns, nullptr, sums)))
-            return s;
-        hipLaunchKernelGGL(k_run_off, gr, blk, 0, ctx->stream, run_idx, len, nruns, run_off);
-    }
-    hipLaunchKernelGGL(k_run_place, g, blk, 0, ctx->stream, cll->keys, ex, run_start, run_off, n,
-                       cll->particle_index);
-    SPHX_HIP(hipGetLastError());
     cll->built = true;
     ++cll->build_count;
     return SPHX_OK;
 }
 
 //---------------------------------------------------------------------------
-// Allocation-free cell-list update: the workspace of the run placement, the
-// radix passes and the scans lives in one file-scope device array of fixed
-// size.  Each device holds its own copy from module load to process end; its
+// Allocation-free cell-list update: the workspace of the update (a cursor
+// per cell and the scan's tile sums) lives in one file-scope device array of
+// fixed size.  Each device holds its own copy from module load to process end; its
 // device address is looked up once per device (noalloc_scratch_init, called
 // by the library's context creation with the device current) and only that
 // address is handed to kernels and copies.  Calls on one device share the
@@ -1632,17 +1509,14 @@ sphx_status cell_linked_list_update(sphx_cell_linked_list *cll, const CllWork *w
 //---------------------------------------------------------------------------
 namespace
 {
-constexpr size_t kNoAllocScanSums = (kNoAllocMaxCells + 1 + kTile - 1) / kTile; // also >= tiles of n
-constexpr size_t kNoAllocHist = (size_t)kBuckets * ((kNoAllocMaxParticles + kTile - 1) / kTile);
-constexpr size_t kNoAllocWords = 8ull * kNoAllocMaxParticles + kNoAllocScanSums + kNoAllocHist;
-static_assert(kNoAllocScanSums >= (kNoAllocMaxParticles + kTile - 1) / kTile, "scan sums cover n");
-static_assert((kNoAllocHist + kTile - 1) / kTile <= kNoAllocScanSums, "scan sums cover the histogram scan");
+constexpr size_t kNoAllocScanSums = (kNoAllocMaxCells + 1 + kTile - 1) / kTile;
+constexpr size_t kNoAllocWords = kNoAllocMaxCells + kNoAllocScanSums;
 constexpr int kMaxDevices = 64;
 std::mutex g_noalloc_mutex;
 uint32_t *g_noalloc_ptr[kMaxDevices] = {};
 } // namespace
 
-// 8 n + scan sums + histogram words for n <= kNoAllocMaxParticles (about 32 MiB)
+// a cursor per cell and the scan's tile sums for at most kNoAllocMaxCells cells (16 MiB)
 static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
 
 sphx_status noalloc_scratch_init(int device)
@@ -1675,21 +1549,16 @@ sphx_status cell_linked_list_update_noalloc(sphx_cell_linked_list *cll)
         set_error("allocation-free workspace not set up on this device (context creation does it)");
         return SPHX_E_STATE;
     }
-    const size_t n = cll->body->n;
-    if (n > kNoAllocMaxParticles || cll->ncells > kNoAllocMaxCells)
+    if (cll->ncells > kNoAllocMaxCells)
     {
-        set_error("allocation-free cell-list update: " + std::to_string(n) + " particles and " +
-                  std::to_string(cll->ncells) + " cells exceed its workspace (at most " +
-                  std::to_string(kNoAllocMaxParticles) + " particles and " + std::to_string(kNoAllocMaxCells) +
+        set_error("allocation-free cell-list update: " + std::to_string(cll->ncells) +
+                  " cells exceed its workspace (at most " + std::to_string(kNoAllocMaxCells) +
                   " cells); use sphx_update_cell_linked_list");
         return SPHX_E_CAPACITY;
     }
     CllWork w;
-    w.runs = base;                              // 6 n
-    w.sort_keys = base + 6ull * kNoAllocMaxParticles; // n
-    w.sort_vals = base + 7ull * kNoAllocMaxParticles; // n
-    w.scan_sums = base + 8ull * kNoAllocMaxParticles;
-    w.hist = w.scan_sums + kNoAllocScanSums;
+    w.cursor = base;                    // one per cell
+    w.scan_sums = base + kNoAllocMaxCells; // tile sums of the cell-offset scan
     return cell_linked_list_update(cll, &w);
 }
 
@@ -1794,7 +1663,7 @@ sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long
     b.needed = needed;
     if (!scan)
     {
-        ProfileScope prof(ctx, "relation_build");
+        ProfileScope prof(ctx, rel->inner ? "relation_build" : "relation_build_contact");
         return rel->source->D == 2 ? launch_tile<2>(ctx, b, nchunks) : launch_tile<3>(ctx, b, nchunks);
     }
     ProfileScope prof(ctx, "relation_scan");

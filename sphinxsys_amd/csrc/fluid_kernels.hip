@@ -221,15 +221,21 @@ struct alignas(4 * sizeof(T)) Coef4
 // are fetched, the distances computed and the 8 kernel intervals requested
 // from LDS before any pair is finished.  fetch(entry) returns a record with
 // the neighbour's position and volume in .pv (entries past the count fetch
-// entry 0, which every non-empty list has, and are not finished); pair(record,
-// geo) finishes a pair.  Pairs are finished in list order with the arithmetic
-// of eval_poly, so the sums are those of the plain walk.
-template <int D, class T, class I, class Fetch, class Pair>
+// entry 0, which every non-empty list has, and are not finished); late(entry,
+// record) loads the rest of the record, issued with the kernel intervals once
+// the positions are no longer needed (so the batch's records and geometry do
+// not hold registers at the same time); pair(record, geo) finishes a pair.
+// Pairs are finished in list order with the arithmetic of eval_poly, so the
+// sums are those of the plain walk.
+template <int D, uint32_t KB = 8, class T, class I, class Fetch, class Late, class Pair>
 __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, uint32_t count,
                                                const PV<T, D> &pi, const PolyLDS<T> &lds, T x_per_r,
-                                               Fetch &&fetch, Pair &&pair)
+                                               Fetch &&fetch, Late &&late, Pair &&pair)
 {
-    constexpr uint32_t kBatch = 8; // 4 (double: 4 waves per SIMD instead of 2) measured slower
+    // pairs per batch: 8; 4 measured slower in double (4 waves per SIMD instead
+    // of 2); 6 is faster for the FP32-mixed pass 2, which spills at 8 under its
+    // register cap
+    constexpr uint32_t kBatch = KB;
     const uint32_t nbatch = (count + kBatch - 1) / kBatch;
     uint32_t next[kBatch];
     auto load_entries = [&](uint32_t b) {
@@ -273,8 +279,11 @@ __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, 
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; ++q)
             c[q] = *reinterpret_cast<const Coef4<T> *>(lds.c[loc[q]]);
-        // keep the 8 reads issued together (the scheduler would otherwise sink
-        // each next to its use and wait on it alone)
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            late(idx[q], rec[q]);
+        // keep the batch's reads issued together (the scheduler would otherwise
+        // sink each next to its use and wait on it alone)
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; ++q)
@@ -431,12 +440,15 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
             cd += (p_i - rj.p) * dWV;
     };
     const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+    auto no_late = [](uint32_t, Rec &) {};
     if (staged)
-        walk_pipelined<D>(a.local_slice, nb, nc, pi, poly_in, x_per_r,
-                          [&](uint32_t r) { return Rec{load_pv<T, D>(s_pv, r), s_p[r]}; }, pair);
+        walk_pipelined<D>(
+            a.local_slice, nb, nc, pi, poly_in, x_per_r,
+            [&](uint32_t r) { return Rec{load_pv<T, D>(s_pv, r), s_p[r]}; }, no_late, pair);
     else
-        walk_pipelined<D>(a.slice, nb, nc, pi, poly_in, x_per_r,
-                          [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.p[j]}; }, pair);
+        walk_pipelined<D>(
+            a.slice, nb, nc, pi, poly_in, x_per_r,
+            [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.p[j]}; }, no_late, pair);
     const T k_avg = T(2) * a.inv_rho0c0_sum * a.rho0c0; // 2 * AverageP / (p_i + p_j)
     const T vol_i = pi.vol;
     T Fx = a.force[0][i], Fy = a.force[1][i], Fz = D == 3 ? a.force[2][i] : T(0);
@@ -576,6 +588,8 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and the velocity components
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
+    // (no opaque base as in pass 1: its extra register makes this pass spill,
+    // 45% slower measured)
     T *s_vx = (T *)(s_pv + kPass2Cap), *s_vy = s_vx + kPass2Cap, *s_vz = s_vy + kPass2Cap;
     const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
     const bool staged = n_staged <= kPass2Cap; // kStageOverflow is above any cap
@@ -640,16 +654,21 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
             }
         };
         const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
+        // whole records up front: loading the velocities late (as pass 1 loads
+        // its pressures) makes this pass spill under its register cap
+        auto no_late = [](uint32_t, Rec &) {};
+        constexpr uint32_t kWalkBatch = P == P_MIXED ? 6u : 8u;
         if (staged)
-            walk_pipelined<D>(a.local_slice, nb, nc, pi, poly_in, x_per_r,
-                              [&](uint32_t r) {
-                                  return Rec{load_pv<T, D>(s_pv, r),
-                                             Rec4<T>{s_vx[r], s_vy[r], D == 3 ? s_vz[r] : T(0), T(0)}};
-                              },
-                              pair);
+            walk_pipelined<D, kWalkBatch>(
+                a.local_slice, nb, nc, pi, poly_in, x_per_r,
+                [&](uint32_t r) {
+                    return Rec{load_pv<T, D>(s_pv, r), Rec4<T>{s_vx[r], s_vy[r], D == 3 ? s_vz[r] : T(0), T(0)}};
+                },
+                no_late, pair);
         else
-            walk_pipelined<D>(a.slice, nb, nc, pi, poly_in, x_per_r,
-                              [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.vel[j]}; }, pair);
+            walk_pipelined<D, kWalkBatch>(
+                a.slice, nb, nc, pi, poly_in, x_per_r,
+                [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.vel[j]}; }, no_late, pair);
         const T C_i = full_C<P>(a.C[i]);
         const T vol_i = pi.vol;
         T dC = a.dC[i] + div * C_i;

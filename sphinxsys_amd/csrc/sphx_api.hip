@@ -39,6 +39,10 @@ namespace
 std::mutex g_mem_mutex;
 std::map<int, std::multimap<size_t, void *>> g_free_blocks; // device -> (bytes, block)
 std::map<void *, std::pair<int, size_t>> g_block_size;      // block -> (device, bytes)
+std::map<int, size_t> g_kept_bytes;                         // device -> bytes in g_free_blocks
+// at most this much released memory is kept per device; beyond it a released
+// block goes back to HIP at once (hipFree, which waits for the device)
+constexpr size_t kKeepCap = 16ull << 30;
 } // namespace
 
 sphx_status dev_alloc(hipStream_t s, void **p, size_t bytes)
@@ -53,6 +57,7 @@ sphx_status dev_alloc(hipStream_t s, void **p, size_t bytes)
         if (it != blocks.end() && it->first / 2 <= bytes)
         {
             *p = it->second;
+            g_kept_bytes[dev] -= it->first;
             blocks.erase(it);
             return SPHX_OK;
         }
@@ -69,6 +74,7 @@ sphx_status dev_alloc(hipStream_t s, void **p, size_t bytes)
             g_block_size.erase(kv.second);
         }
         g_free_blocks[dev].clear();
+        g_kept_bytes[dev] = 0;
         e = hipMalloc(p, bytes);
     }
     if (e != hipSuccess)
@@ -85,8 +91,18 @@ void dev_free(hipStream_t s, void *p)
     (void)hipStreamSynchronize(s);
     std::lock_guard<std::mutex> lock(g_mem_mutex);
     auto it = g_block_size.find(p);
-    if (it != g_block_size.end())
-        g_free_blocks[it->second.first].emplace(it->second.second, p);
+    if (it == g_block_size.end())
+        return;
+    const int dev = it->second.first;
+    const size_t bytes = it->second.second;
+    if (g_kept_bytes[dev] + bytes > kKeepCap)
+    {
+        (void)hipFree(p);
+        g_block_size.erase(it);
+        return;
+    }
+    g_kept_bytes[dev] += bytes;
+    g_free_blocks[dev].emplace(bytes, p);
 }
 
 // streams and pinned scalar mirrors of destroyed contexts, reused by the next
@@ -579,7 +595,7 @@ sphx_status sphx_context_create(int device, sphx_context **out)
     if (!ctx->stream)
         SPHX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (Scratch *sc : {&ctx->sort_keys, &ctx->sort_vals, &ctx->sort_hist, &ctx->scan_blocks, &ctx->reduce_parts,
-                        &ctx->gather_tmp, &ctx->cll_runs})
+                        &ctx->gather_tmp, &ctx->cll_cursor})
         sc->stream = ctx->stream;
     if (!ctx->scalars_h)
         SPHX_HIP(hipHostMalloc(&ctx->scalars_h, kScalarSlots * sizeof(unsigned long long), hipHostMallocDefault));
@@ -604,7 +620,7 @@ sphx_status sphx_context_destroy(sphx_context *ctx)
     for (hipEvent_t e : ctx->event_pool)
         (void)hipEventDestroy(e);
     for (Scratch *sc : {&ctx->sort_keys, &ctx->sort_vals, &ctx->sort_hist, &ctx->scan_blocks, &ctx->reduce_parts,
-                        &ctx->gather_tmp, &ctx->cll_runs})
+                        &ctx->gather_tmp, &ctx->cll_cursor})
         sc->release();
     dev_free(ctx->stream, ctx->scalars_d);
     (void)hipStreamSynchronize(ctx->stream);
@@ -655,6 +671,30 @@ sphx_status sphx_profile_read(sphx_context *ctx, const char *stage, double *tota
     return SPHX_OK;
 }
 
+namespace
+{
+// device blocks allocated by a create function, given back (dev_free) unless
+// the function succeeds and calls keep()
+struct AllocGuard
+{
+    hipStream_t stream;
+    std::vector<void *> blocks;
+    bool kept = false;
+    void add(void *p)
+    {
+        if (p)
+            blocks.push_back(p);
+    }
+    void keep() { kept = true; }
+    ~AllocGuard()
+    {
+        if (!kept)
+            for (void *p : blocks)
+                sphx::dev_free(stream, p);
+    }
+};
+} // namespace
+
 sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_body **out)
 {
     if (!ctx || !d || !out)
@@ -680,6 +720,17 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
     const size_t T = b->P == P_FP64 ? sizeof(double) : sizeof(float);
     const size_t R4 = b->P == P_FP64 ? sizeof(Rec4<double>) : sizeof(Rec4<float>);
     b->tsize = T;
+    // on a failure below, the blocks allocated so far go back to the cache
+    struct Undo
+    {
+        sphx_body *b;
+        ~Undo()
+        {
+            if (b)
+                for (void *p : b->allocations)
+                    dev_free(b->ctx->stream, p);
+        }
+    } undo{b.get()};
     sphx_status s;
     for (int k = 0; k < 3; ++k)
     {
@@ -769,6 +820,7 @@ sphx_status sphx_body_create(sphx_context *ctx, const sphx_body_desc *d, sphx_bo
                 SPHX_HIP(hipMemsetAsync(f, 0, nf * T, ctx->stream));
     }
     SPHX_HIP(hipStreamSynchronize(ctx->stream));
+    undo.b = nullptr;
     *out = b.release();
     return SPHX_OK;
 }
@@ -885,10 +937,18 @@ sphx_status sphx_cell_linked_list_create(sphx_body *b, const double *lo, const d
     SPHX_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     sphx_status st;
-    if ((st = dev_alloc(s, (void **)&c->cell_offset, (ncells + 1) * sizeof(uint32_t))) ||
-        (st = dev_alloc(s, (void **)&c->particle_index, std::max<size_t>(b->n, 1) * sizeof(uint32_t))) ||
-        (st = dev_alloc(s, (void **)&c->keys, std::max<size_t>(b->n, 1) * sizeof(uint32_t))))
+    AllocGuard guard{s};
+    st = dev_alloc(s, (void **)&c->cell_offset, (ncells + 1) * sizeof(uint32_t));
+    guard.add(c->cell_offset);
+    if (!st)
+        st = dev_alloc(s, (void **)&c->particle_index, std::max<size_t>(b->n, 1) * sizeof(uint32_t));
+    guard.add(c->particle_index);
+    if (!st)
+        st = dev_alloc(s, (void **)&c->keys, std::max<size_t>(b->n, 1) * sizeof(uint32_t));
+    guard.add(c->keys);
+    if (st)
         return st;
+    guard.keep();
     *out = c.release();
     return SPHX_OK;
 }
@@ -989,6 +1049,21 @@ static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_lin
     r->inv_h = 1.0 / std::max(src->h, tar->h);
     SPHX_HIP(hipSetDevice(src->ctx->device));
     hipStream_t stream = src->ctx->stream;
+    // on a failure below, the blocks allocated so far go back to the cache
+    struct Undo
+    {
+        sphx_relation *r;
+        hipStream_t s;
+        ~Undo()
+        {
+            if (!r)
+                return;
+            for (void *p : {(void *)r->neighbor_size, (void *)r->particle_offset, (void *)r->chunk_offset,
+                            (void *)r->stage_count, (void *)r->stage_keys, r->poly})
+                if (p)
+                    dev_free(s, p);
+        }
+    } undo{r.get(), stream};
     {
         sphx_status st;
         if ((st = dev_alloc(stream, (void **)&r->neighbor_size, (size_t)(src->n + 1) * sizeof(uint32_t))) ||
@@ -1018,6 +1093,7 @@ static sphx_status relation_create(sphx_body *src, sphx_body *tar, sphx_cell_lin
     sphx_status s = build_kernel_poly(r.get());
     if (s)
         return s;
+    undo.r = nullptr;
     *out = r.release();
     return SPHX_OK;
 }

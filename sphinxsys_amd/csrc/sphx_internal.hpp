@@ -89,7 +89,7 @@ struct sphx_context
     std::map<std::string, std::pair<double, uint64_t>> profile; // stage -> (ms, launches)
     unsigned long long *scalars_d = nullptr; // kScalarSlots u64 on the device
     unsigned long long *scalars_h = nullptr; // pinned mirror
-    sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp, cll_runs;
+    sphx::Scratch sort_keys, sort_vals, sort_hist, scan_blocks, reduce_parts, gather_tmp, cll_cursor;
     // while set, the acoustic kernels take dt and their run flag from the SLOT_STEP slots
     bool device_step = false;
     uint32_t last_substeps = 0; // sub-steps of the previous sphx_advection_step
@@ -195,26 +195,21 @@ struct sphx_relation
 namespace sphx
 {
 // ---- configuration kernels (config_kernels.hip) ----
-// Caller-provided device workspace of the cell-list update (null members: the
-// context's grow-only scratch is used and grown with dev_alloc).  The caller
-// guarantees the sizes: runs 6 n, sort_keys and sort_vals n, scan_sums the
-// tile count of the longest scan, hist kBuckets x tiles of n (words of 4 B).
+// Caller-provided device workspace of the cell-list update (null: the
+// context's grow-only scratch is used and grown with dev_alloc): a cursor
+// per cell and the tile sums of the cell-offset scan (words of 4 B).
 struct CllWork
 {
-    uint32_t *runs = nullptr;
-    uint32_t *sort_keys = nullptr, *sort_vals = nullptr;
+    uint32_t *cursor = nullptr;
     uint32_t *scan_sums = nullptr;
-    uint32_t *hist = nullptr;
 };
 sphx_status exclusive_scan_u32(sphx_context *ctx, const uint32_t *in, uint32_t *out, size_t n,
                                uint32_t *total_host, uint32_t *sums = nullptr);
-sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits,
-                             const CllWork *work = nullptr);
+sphx_status radix_sort_pairs(sphx_context *ctx, uint32_t *keys, uint32_t *vals, size_t n, int bits);
 sphx_status cell_linked_list_update(sphx_cell_linked_list *cll, const CllWork *work = nullptr);
-// The allocation-free cell-list update: limits of its file-scope device
-// workspace, its per-device address (looked up once per device by
+// The allocation-free cell-list update: the cell limit of its file-scope
+// device workspace, its per-device address (looked up once per device by
 // noalloc_scratch_init, with that device current) and the update itself
-constexpr uint32_t kNoAllocMaxParticles = 1u << 20;
 constexpr uint64_t kNoAllocMaxCells = 1ull << 22;
 sphx_status noalloc_scratch_init(int device);
 sphx_status cell_linked_list_update_noalloc(sphx_cell_linked_list *cll);

@@ -1,9 +1,8 @@
 """sphx_update_cell_linked_list_noalloc (the allocation-free cell-list update,
 workspace in a fixed device array of the library) against
 sphx_update_cell_linked_list: bit-identical cell offsets and particle indices
-on two calls in a row, the second on a smaller body, both paths of the run
-placement (many runs: the radix passes; few runs: the one-workgroup sort),
-and the capacity check."""
+on two calls in a row, the second on a smaller body (particles in random
+order, then in cell order), and the capacity check at the cell limit."""
 import ctypes as C
 
 import numpy as np
@@ -14,7 +13,7 @@ from sphinxsys_amd import ck
 
 pytestmark = pytest.mark.gpu
 
-MAX_PARTICLES = 1 << 20  # the workspace limit stated in include/sphx.h
+MAX_CELLS = 1 << 22  # the workspace limit stated in include/sphx.h
 
 
 def lattice(dim, m, dp):
@@ -55,27 +54,33 @@ def check_case(hip, oracle, pos, dp):
 
 def test_noalloc_matches_allocating_update_large_then_small(hip, oracle):
     rng = np.random.default_rng(20261203)
-    # first call: 300k particles in random order (as many runs as particles:
-    # the radix passes), 3D
+    # first call: 300k particles in random order, 3D
     big = rng.uniform(-0.1, 1.1, size=(300000, 3))
     print("no-alloc cell list: 300000 random 3D particles", flush=True)
     check_case(hip, oracle, big, 0.004)
-    # second call, smaller: a 2D lattice in cell order (few runs: one workgroup)
+    # second call, smaller: a 2D lattice in cell order
     small = lattice(2, 70, 0.0125)
     print("no-alloc cell list: 4900 lattice 2D particles", flush=True)
     check_case(hip, oracle, small, 0.0125)
 
 
-def test_noalloc_at_the_particle_limit_and_beyond(hip):
+def test_noalloc_at_the_cell_limit_and_beyond(hip):
+    """A mesh of exactly the workspace's cell count works and equals the
+    allocating update; one more row of cells is refused before any launch."""
     rng = np.random.default_rng(7)
-    dim = 2
-    for n, ok in ((MAX_PARTICLES, True), (MAX_PARTICLES + 1, False)):
-        pos = rng.uniform(0.0, 1.0, size=(n, dim))
+    dim, dp = 2, 0.0025
+    gs = 2.0 * 1.3 * dp
+    # the mesh spans ceil(extent / gs) + 4 cells per axis (2-cell buffer each side)
+    per_axis = int(MAX_CELLS ** 0.5)  # 2048 x 2048 = the limit exactly
+    for cells_y, ok in ((per_axis, True), (per_axis + 1, False)):
+        ext_x, ext_y = (per_axis - 4) * gs, (cells_y - 4) * gs
+        pos = rng.uniform((0.0, 0.0), (ext_x, ext_y), size=(200000, dim))
         ctx = ck.Context(hip)
-        b = ck.Body(ctx, dim, n, 0.002)
+        b = ck.Body(ctx, dim, len(pos), dp)
         b.write("Position", pos)
-        c = ck.CellLinkedList(b, np.zeros(dim), np.ones(dim))
+        c = ck.CellLinkedList(b, np.zeros(dim), np.array([ext_x, ext_y]), grid_spacing=gs)
         try:
+            assert (c.n_cells <= MAX_CELLS) == ok, (c.all_cells, c.n_cells)
             status = hip.lib.sphx_update_cell_linked_list_noalloc(c.h)
             if ok:
                 assert status == capi.SPHX_OK, hip.lib.sphx_last_error().decode()
