@@ -81,18 +81,55 @@ def profiled_traffic(stage):
     return None
 
 
-def timed_window(sc, steps):
+class ProfileWindow:
+    """Host timestamps around chosen regions of the run, for picking their
+    kernel dispatches out of a rocprofv3 trace or counter collection of the
+    whole run: CLOCK_BOOTTIME in ns, the clock of rocprofv3's dispatch
+    timestamps, read after a synchronisation on both sides, so a dispatch of
+    the region starts after the first stamp and ends before the second.  Each
+    region is printed to stderr as "profile-window NAME T0 T1" and returned in
+    the JSON line (sphinxsys_amd.profile_summary reads either).  Off, both
+    calls do nothing."""
+
+    def __init__(self, on, name="headline"):
+        self.on, self.name, self.spans = on, name, {}
+        self._t0 = None
+
+    @staticmethod
+    def now():
+        return time.clock_gettime_ns(time.CLOCK_BOOTTIME)
+
+    def resume(self, name=None):
+        if self.on:
+            self._t0 = (name or self.name, self.now())
+
+    def pause(self):
+        if self.on and self._t0 is not None:
+            name, t0 = self._t0
+            t1 = self.now()
+            self.spans[name] = [t0, t1]
+            log(f"profile-window {name} {t0} {t1}")
+            self._t0 = None
+
+
+NO_WINDOW = ProfileWindow(False)
+
+
+def timed_window(sc, steps, collect=NO_WINDOW):
     """K advection steps timed by the host clock (synchronised on both sides),
     with HIP events around the two acoustic passes only (events around every
-    small kernel would slow the step by 2-3 %)."""
+    small kernel would slow the step by 2-3 %); `collect` brackets the steps
+    for a profiler."""
     sc.ctx.synchronize()
     sc.ctx.profile(True, passes_only=True)
     a0, i0, t_phys = sc.acoustic_steps, sc.iterations, sc.time
+    collect.resume()
     t0 = time.perf_counter()
     for _ in range(steps):
         sc.advection_step()
     sc.ctx.synchronize()
     wall = time.perf_counter() - t0
+    collect.pause()
     passes = {k: sc.ctx.profile_read(k) for k in ("acoustic_pass1", "acoustic_pass2")}
     sc.ctx.profile(False)
     return {"wall_s": wall, "acoustic_steps": sc.acoustic_steps - a0, "advection_steps": sc.iterations - i0,
@@ -116,12 +153,13 @@ def stage_breakdown(sc, steps):
     return prof
 
 
-def run_gpu(lib, geom, precision, warmup, steps, sustain_t=0.0):
+def run_gpu(lib, geom, precision, warmup, steps, sustain_t=0.0, collect=NO_WINDOW):
     """The scene on the GPU: a fresh-lattice window (after `warmup` steps) and,
     with sustain_t > 0, a mid-run window after the run has passed physical time
     sustain_t (started 40 advection steps after a ParticleSortCK, so that no
     sort falls inside it); then one ParticleSortCK timed alone for the amortised
-    cost (the CK 3D case sorts every 100 advection steps, dambreak.cpp:213-216)."""
+    cost (the CK 3D case sorts every 100 advection steps, dambreak.cpp:213-216).
+    `collect` brackets the headline window (the mid-run one when there is one)."""
     from sphinxsys_amd import cases
     t0 = time.time()
     sc = cases.DamBreak(lib, geom, precision)
@@ -129,7 +167,8 @@ def run_gpu(lib, geom, precision, warmup, steps, sustain_t=0.0):
     log(f"  scene ready in {time.time() - t0:.1f}s ({sc.fluid.n} fluid, {sc.wall.n} wall)")
     for _ in range(warmup):
         sc.advection_step()
-    out = {"n_fluid": sc.fluid.n, "n_wall": sc.wall.n, "fresh": timed_window(sc, steps)}
+    out = {"n_fluid": sc.fluid.n, "n_wall": sc.wall.n,
+           "fresh": timed_window(sc, steps, collect if sustain_t <= 0.0 else NO_WINDOW)}
     window = out["fresh"]
     if sustain_t > 0.0:
         t0 = time.perf_counter()
@@ -140,7 +179,7 @@ def run_gpu(lib, geom, precision, warmup, steps, sustain_t=0.0):
                 last = time.perf_counter()
                 log(f"  t = {sc.time:.4f} s after {sc.iterations} advection steps, {last - t0:.0f} s")
         log(f"  mid-run window from t = {sc.time:.4f} s, advection step {sc.iterations}")
-        out["sustained"] = timed_window(sc, steps)
+        out["sustained"] = timed_window(sc, steps, collect)
         window = out["sustained"]
     out["profile"] = stage_breakdown(sc, steps)
     inner_size, _, _ = sc.inner.read()
@@ -252,6 +291,57 @@ def compare_runs(a, b):
     }
 
 
+def traversal_ab(lib, geom, warmup, calls):
+    """The second acoustic half step on the stored lists against the same step
+    with the inner neighbours found by a cell traversal, on two copies of the
+    scene advanced identically (deterministic kernels: bit-identical states):
+    the first call's outputs are compared bit for bit, then `calls` calls of
+    each are timed with HIP events on the library's stream, alternating."""
+    import numpy as np
+    from sphinxsys_amd import _capi as capi
+    from sphinxsys_amd import cases, ck
+    scenes = []
+    for _ in range(2):
+        sc = cases.DamBreak(lib, geom, capi.MIXED)
+        sc.initialize()
+        for _ in range(warmup):
+            sc.advection_step()
+        sc.ctx.synchronize()
+        scenes.append(sc)
+    a, b = scenes
+    steps = {"lists": ck.AcousticStep2ndHalf(a.inner, a.contact),
+             "traversal": ck.AcousticStep2ndHalfTraversal(b.inner, b.contact)}
+    owner = {"lists": a, "traversal": b}
+    dt = 1.0e-5
+    for name in steps:
+        steps[name].exec(dt)
+        owner[name].ctx.synchronize()
+    outputs = ("Force", "CompressionRate", "Compression", "Density", "Displacement")
+    identical = all(np.array_equal(a.fluid.read(k), b.fluid.read(k)) for k in outputs)
+    ms = {name: [] for name in steps}
+    for _ in range(calls):
+        for name, step in steps.items():
+            ctx = owner[name].ctx
+            ctx.profile(True, passes_only=True)
+            step.exec(dt)
+            ctx.synchronize()
+            t, n = ctx.profile_read("acoustic_pass2")
+            ctx.profile(False)
+            ms[name].append(t / max(n, 1))
+    inner_size, _, _ = a.inner.read()
+    out = {"metric": "second acoustic half step, stored lists against cell traversal (pass-2 kernel ms)",
+           "workload": f"3D dam break, {a.fluid.n} fluid + {a.wall.n} wall particles, FP32-mixed, "
+                       f"after {warmup} advection steps",
+           "pass2_ms": {k: float(np.median(v)) for k, v in ms.items()},
+           "pass2_ms_min": {k: float(np.min(v)) for k, v in ms.items()},
+           "calls": calls, "outputs_bit_identical": identical,
+           "mean_inner_neighbors": float(inner_size.mean())}
+    out["traversal_over_lists"] = out["pass2_ms"]["traversal"] / out["pass2_ms"]["lists"]
+    for sc in scenes:
+        sc.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, repeats=3):
     """Plain device-to-device copies of a known size through the HIP runtime:
     the reference reading of the HBM counters (FETCH_SIZE / WRITE_SIZE)."""
@@ -283,6 +373,9 @@ def main():
                     help="physical time after which the mid-run (headline) window is timed; 0: fresh only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--profile-window", action="store_true",
+                    help="print host timestamps of the headline window of the FP32-mixed run (and of the "
+                         "copy calibration) for python -m sphinxsys_amd.profile_summary")
     ap.add_argument("--full-run", type=float, default=0.0, metavar="T_END",
                     help="instead of the throughput line: run the scene to T_END in FP32-mixed and FP64 "
                          "and compare probe pressure and surge front (one JSON line)")
@@ -295,6 +388,9 @@ def main():
                          "(20k-2M particles, t_end 1.0 s); p50/p99 time to solution")
     ap.add_argument("--rate", type=float, default=4.0, help="arrival rate of --served (requests/s)")
     ap.add_argument("--slots", type=int, default=256, help="concurrent scenes of --batched / --served")
+    ap.add_argument("--traversal-ab", action="store_true",
+                    help="instead of the throughput line: the second acoustic half step on the stored lists "
+                         "against the cell traversal, on the same scene (bit-identity and pass-2 ms)")
     ap.add_argument("--microbench", type=int, default=0, metavar="M",
                     help="instead of the throughput line: the neighbour-search + density-summation "
                          "microbenchmark on an M^3 jittered lattice (M=256: 16.8M particles)")
@@ -309,8 +405,11 @@ def main():
     from sphinxsys_amd import cases
 
     lib = sphinxsys_amd.load()
+    collect = ProfileWindow(args.profile_window)
     if args.copy_calibration:
+        collect.resume("copy_calibration")
         copy_calibration()
+        collect.pause()
     t0 = time.time()
     geom = cases.dambreak_3d(dp=args.dp)
     log(f"geometry: {len(geom.fluid_pos)} fluid, {len(geom.wall_pos)} wall in {time.time() - t0:.1f}s")
@@ -353,6 +452,9 @@ def main():
                           "same_steps_and_time_alone_and_in_mix": same,
                           "batch_tuning": server.tuning}), flush=True)
         return
+    if args.traversal_ab:
+        print(json.dumps(traversal_ab(lib, geom, args.warmup, args.steps)), flush=True)
+        return
     if args.microbench > 0:
         import subprocess
         exe = os.path.join(ROOT, "sphinxsys_amd", "lib", "microbench")
@@ -384,7 +486,7 @@ def main():
         return
 
     log("FP32-mixed run")
-    mixed = run_gpu(lib, geom, capi.MIXED, args.warmup, args.steps, sustain_t=args.sustain)
+    mixed = run_gpu(lib, geom, capi.MIXED, args.warmup, args.steps, sustain_t=args.sustain, collect=collect)
     fp64 = None
     if not args.no_fp64:
         log("FP64 run")
@@ -483,6 +585,7 @@ def main():
         "stage_ms_source": f"HIP events around every launch over the {args.steps} advection steps after the "
                            f"headline window (the timed steps time only the two acoustic passes)",
         "observables": mixed["observables"],
+        **({"profile_windows": collect.spans} if collect.on else {}),
     }
     if fp64:
         v64 = rate(fp64, "fresh")[0]

@@ -224,6 +224,14 @@ sphx_status sphx_acoustic_step_1st_half(sphx_relation *inner, sphx_relation *con
 /* reference: ... AcousticStep2ndHalf<...> (fluid_dynamics/acoustic_step_2nd_half.hpp:33-137). */
 sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *const *walls,
                                         int n_wall, int riemann, double dt);
+/* No counterpart in the reference: sphx_acoustic_step_2nd_half with the inner neighbours
+ * found by a cell traversal (one wave per 64-particle chunk, the chunk's search-box cells
+ * staged in LDS, the neighbour criterion evaluated per candidate) instead of read from the
+ * stored lists, for the comparison of the two designs. Same results bit for bit (the
+ * neighbours are taken in list order); 3D FP32-mixed bodies only (SPHX_E_ARG otherwise).
+ * The inner relation's cell linked list must be current. */
+sphx_status sphx_acoustic_step_2nd_half_traversal(sphx_relation *inner, sphx_relation *const *walls,
+                                                  int n_wall, int riemann, double dt);
 /* reference: InteractionDynamicsCK<P, LinearCorrectionMatrixComplex> built with
  * DynamicsArgs(inner, alpha) and the wall contact (general_dynamics/kernel_correction_ck.hpp:37-90,
  * test_3d_dambreak_sycl/dambreak.cpp:123-124): B_i = -sum_j r_ij (x) grad W_ij V_j over the

@@ -1146,6 +1146,13 @@ sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *con
 {
     return second_half(inner, walls, n, riemann, dt, false);
 }
+// the device library's cell-traversal form has the neighbours of the lists: on
+// the CPU it is the 2nd half itself
+sphx_status sphx_acoustic_step_2nd_half_traversal(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                                  int riemann, double dt)
+{
+    return second_half(inner, walls, n, riemann, dt, false);
+}
 sphx_status sphx_acoustic_step_1st_half_corrected(sphx_relation *inner, sphx_relation *const *walls, int n,
                                                   int riemann, double dt)
 {

@@ -41,7 +41,8 @@ SYMBOLS = [
     "sphx_update_relation", "sphx_relation_read", "sphx_particle_sort", "sphx_gravity_force",
     "sphx_density_regularization", "sphx_advection_step_setup",
     "sphx_update_particle_position", "sphx_compression_summation",
-    "sphx_acoustic_step_1st_half", "sphx_acoustic_step_2nd_half", "sphx_acoustic_time_step",
+    "sphx_acoustic_step_1st_half", "sphx_acoustic_step_2nd_half", "sphx_acoustic_step_2nd_half_traversal",
+    "sphx_acoustic_time_step",
     "sphx_advection_time_step", "sphx_total_mechanical_energy", "sphx_upper_front_in_axis",
     "sphx_interpolate_contact", "sphx_advection_step", "sphx_profile_enable",
     "sphx_profile_read", "sphx_linear_correction_matrix",
@@ -119,6 +120,7 @@ class SphxLibrary:
             "sphx_compression_summation": (C.c_int, [_vp, _pp, C.c_int]),
             "sphx_acoustic_step_1st_half": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
             "sphx_acoustic_step_2nd_half": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
+            "sphx_acoustic_step_2nd_half_traversal": (C.c_int, [_vp, _pp, C.c_int, C.c_int, C.c_double]),
             "sphx_acoustic_time_step": (C.c_int, [_vp, C.c_double, _dp]),
             "sphx_advection_time_step": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
             "sphx_total_mechanical_energy": (C.c_int, [_vp, _dp, _dp]),

@@ -270,6 +270,17 @@ class AcousticStep2ndHalf(AcousticStep1stHalf):
     _entry = ("sphx_acoustic_step_2nd_half", "sphx_acoustic_step_2nd_half_corrected")
 
 
+class AcousticStep2ndHalfTraversal(AcousticStep1stHalf):
+    """AcousticStep2ndHalf with the inner neighbours found by a cell traversal
+    instead of read from the stored lists (3D FP32-mixed, uncorrected; the same
+    results bit for bit): the alternative design, kept for its comparison."""
+
+    _entry = ("sphx_acoustic_step_2nd_half_traversal", None)
+
+    def __init__(self, inner, *walls, riemann=capi.RIEMANN_ACOUSTIC):
+        super().__init__(inner, *walls, riemann=riemann)
+
+
 def AcousticStep1stHalfWithWallRiemannCK(inner, *walls):
     return AcousticStep1stHalf(inner, *walls, riemann=capi.RIEMANN_ACOUSTIC)
 

@@ -255,25 +255,8 @@ __global__ void __launch_bounds__(kThreads) k_radix_scatter(const uint32_t *keys
 }
 
 //---------------------------------------------------------------------------
-// cell linked list: Mesh::LinearCellIndexFromPosition, base_mesh.hxx:9-26,73-78
+// cell keys and sort keys (the cell index of a position: cell_index, linear_cell)
 //---------------------------------------------------------------------------
-template <int D>
-__device__ __forceinline__ void cell_index(const MeshParams &m, const double *x, int *c)
-{
-#pragma unroll
-    for (int k = 0; k < D; ++k)
-    {
-        int ci = (int)floor((x[k] - m.lower[k]) / m.gs);
-        c[k] = min(max(ci, 0), m.all_grid_points[k] - 2);
-    }
-}
-template <int D>
-__device__ __forceinline__ uint32_t linear_cell(const MeshParams &m, const int *c)
-{
-    if (D == 2)
-        return (uint32_t)(c[0] * m.all_cells[1] + c[1]);
-    return (uint32_t)(c[0] * m.all_cells[1] * m.all_cells[2] + c[1] * m.all_cells[2] + c[2]);
-}
 // Mesh::MortonCode, base_mesh.hxx:91-99
 __device__ __forceinline__ uint32_t morton_code(uint32_t x)
 {
@@ -285,19 +268,6 @@ __device__ __forceinline__ uint32_t morton_code(uint32_t x)
     return x;
 }
 
-struct PosPtrs
-{
-    const double *x, *y, *z;
-};
-
-template <int D>
-__device__ __forceinline__ void load_pos(const PosPtrs &p, uint32_t i, double *x)
-{
-    x[0] = p.x[i];
-    x[1] = p.y[i];
-    if (D == 3)
-        x[2] = p.z[i];
-}
 
 template <int D>
 __device__ __forceinline__ void cell_keys_body(const PosPtrs &pos, uint32_t n, const MeshParams &m, uint32_t *keys,
@@ -457,20 +427,6 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 constexpr uint32_t kRing = 16;      // ring rows per wave
 constexpr uint32_t kFlushEvery = 8; // candidate iterations between flushes
 
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
-    return v;
-}
 
 struct BuildArgs
 {
@@ -1586,6 +1542,24 @@ sphx_status cell_near_flags(sphx_cell_linked_list *cll)
     return SPHX_OK;
 }
 
+// pre-filter bounds: float rounding of positions bounded by |x| <= xmax
+// (target mesh extent + 2.1 h) gives a relative error of the squared
+// distance below 2.2e-7 xmax/h + 5.6e-7 near the cut-off; the margin is 4x that
+void relation_prefilter(const sphx_relation *rel, float *thr_lo, float *thr_hi, double *pre_xmax)
+{
+    const MeshParams &m = rel->cll->mesh;
+    double X = 0.0;
+    for (int k = 0; k < rel->source->D; ++k)
+        X = std::max(X, std::max(std::fabs(m.lower[k]), std::fabs(m.lower[k] + m.all_cells[k] * m.gs)));
+    const double h = 1.0 / rel->inv_h;
+    const double xmax = X + 2.1 * h;
+    const double margin = 4.0 * (2.2e-7 * xmax / h + 5.6e-7) + 1e-6;
+    const double c2 = 4.0 / (rel->inv_h * rel->inv_h);
+    *thr_lo = (float)(c2 * (1.0 - margin));
+    *thr_hi = (float)(c2 * (1.0 + margin));
+    *pre_xmax = X;
+}
+
 static RelationArgs relation_args(sphx_relation *rel)
 {
     RelationArgs a;
@@ -1601,22 +1575,9 @@ static RelationArgs relation_args(sphx_relation *rel)
     a.chunk_offset = rel->chunk_offset;
     a.slice = rel->slice;
     a.near = rel->inner ? nullptr : rel->cll->near;
-    // pre-filter bounds: float rounding of positions bounded by |x| <= xmax
-    // (target mesh extent + 2.1 h) gives a relative error of the squared
-    // distance below 2.2e-7 xmax/h + 5.6e-7 near the cut-off; the margin is 4x that
-    const MeshParams &m = rel->cll->mesh;
-    double X = 0.0;
-    for (int k = 0; k < rel->source->D; ++k)
-        X = std::max(X, std::max(std::fabs(m.lower[k]), std::fabs(m.lower[k] + m.all_cells[k] * m.gs)));
-    const double h = 1.0 / rel->inv_h;
-    const double xmax = X + 2.1 * h;
-    const double margin = 4.0 * (2.2e-7 * xmax / h + 5.6e-7) + 1e-6;
-    const double c2 = 4.0 / (rel->inv_h * rel->inv_h);
     a.src_f = rel->source->pos_f32;
     a.tar_f = rel->target->pos_f32;
-    a.thr_lo = (float)(c2 * (1.0 - margin));
-    a.thr_hi = (float)(c2 * (1.0 + margin));
-    a.pre_xmax = X;
+    relation_prefilter(rel, &a.thr_lo, &a.thr_hi, &a.pre_xmax);
     return a;
 }
 

@@ -215,6 +215,28 @@ struct alignas(4 * sizeof(T)) Coef4
     T c0, c1, c2, c3;
 };
 
+// r_ij = x_i - x_j, 1/|r_ij| and the kernel table's interval and fraction at
+// |r_ij|; the kernel is the interval's cubic at the fraction (pair_kernel),
+// evaluated once the interval's coefficients are read
+template <int D, class T>
+__device__ __forceinline__ void pair_geometry(const PV<T, D> &pi, const PV<T, D> &pj, T x_per_r, PairGeo<T> &g,
+                                              int &loc, T &f)
+{
+    g.dx = pi.x - pj.x;
+    g.dy = pi.y - pj.y;
+    g.dz = D == 3 ? pi.z - pj.z : T(0);
+    const T r2 = D == 3 ? g.dx * g.dx + g.dy * g.dy + g.dz * g.dz : g.dx * g.dx + g.dy * g.dy;
+    g.ir = inv_sqrt(r2);
+    const T x = (r2 * g.ir) * x_per_r;
+    loc = min((int)x, kPolyIntervals - 1);
+    f = frac_part(x);
+}
+template <class T>
+__device__ __forceinline__ T pair_kernel(const Coef4<T> &c, T f)
+{
+    return ((c.c3 * f + c.c2) * f + c.c1) * f + c.c0;
+}
+
 // The neighbour walk of for_each_neighbor with the latencies overlapped: the
 // entries are taken in batches of 8; the list entries of the next batch are
 // loaded while the current one is evaluated, and for each batch the records
@@ -233,8 +255,8 @@ __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, 
                                                Fetch &&fetch, Late &&late, Pair &&pair)
 {
     // pairs per batch: 8; 4 measured slower in double (4 waves per SIMD instead
-    // of 2); 6 is faster for the FP32-mixed pass 2, which spills at 8 under its
-    // register cap
+    // of 2); 6 is faster for the FP32-mixed passes: pass 2 spills at 8 under its
+    // register cap, pass 1 at 6 fits five waves per SIMD
     constexpr uint32_t kBatch = KB;
     const uint32_t nbatch = (count + kBatch - 1) / kBatch;
     uint32_t next[kBatch];
@@ -263,18 +285,7 @@ __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, 
         T f[kBatch];
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; ++q)
-        {
-            const PV<T, D> &pj = rec[q].pv;
-            g[q].dx = pi.x - pj.x;
-            g[q].dy = pi.y - pj.y;
-            g[q].dz = D == 3 ? pi.z - pj.z : T(0);
-            const T r2 = D == 3 ? g[q].dx * g[q].dx + g[q].dy * g[q].dy + g[q].dz * g[q].dz
-                                : g[q].dx * g[q].dx + g[q].dy * g[q].dy;
-            g[q].ir = inv_sqrt(r2);
-            const T x = (r2 * g[q].ir) * x_per_r;
-            loc[q] = min((int)x, kPolyIntervals - 1);
-            f[q] = frac_part(x);
-        }
+            pair_geometry<D>(pi, rec[q].pv, x_per_r, g[q], loc[q], f[q]);
         Coef4<T> c[kBatch];
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; ++q)
@@ -289,7 +300,7 @@ __device__ __forceinline__ void walk_pipelined(const I *entries, uint32_t base, 
         for (uint32_t q = 0; q < kBatch; ++q)
             if (FULL || q < rem)
             {
-                g[q].w = ((c[q].c3 * f[q] + c[q].c2) * f[q] + c[q].c1) * f[q] + c[q].c0;
+                g[q].w = pair_kernel(c[q], f[q]);
                 pair(rec[q], g[q]);
             }
     };
@@ -386,7 +397,20 @@ struct Mat3
 // = k (p_i B_j + p_j B_i) e_ij is summed as p_i grad_sum_i + B_i sum_j p_j dW V_j e_ij
 // (grad_sum_i = sum_j B_j e_ij dW V_j is fixed while positions and volumes are),
 // the wall term as B_i sum_w (p_i + p_w) dW V_w e.
+// The FP32-mixed pass stages at most kPass1Cap records (20 B each: 28 KB and
+// the static tables, five blocks a CU) and walks six pairs per batch (94
+// VGPRs: five waves per SIMD, no spills); a block with a larger staged set
+// reads its neighbours from global memory in this pass.
 //---------------------------------------------------------------------------
+constexpr uint32_t kPass1Cap = 1400;
+static_assert(kPass1Cap <= kStageCap, "pass 1 stages a prefix of the staging capacity");
+
+template <int P>
+__host__ __device__ constexpr uint32_t pass1_cap()
+{
+    return P == P_MIXED ? kPass1Cap : kStageCap;
+}
+
 template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
 __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> &a, const uint32_t blk, double dt_d)
 {
@@ -402,9 +426,9 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
     const uint32_t i = blk * kThreads + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and p of the block's distinct neighbours
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
-    T *s_p = (T *)(s_pv + kStageCap);
+    T *s_p = (T *)(s_pv + pass1_cap<P>());
     const uint32_t n_staged = a.stage_count ? a.stage_count[blk] : kStageOverflow;
-    const bool staged = n_staged != kStageOverflow;
+    const bool staged = n_staged <= pass1_cap<P>(); // kStageOverflow is above any cap
     if (staged)
     {
         const uint32_t *keys = a.stage_keys + (size_t)blk * a.stage_cap;
@@ -441,12 +465,13 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
     };
     const uint32_t nb = a.chunk[i >> 6] + (i & 63u), nc = a.count[i];
     auto no_late = [](uint32_t, Rec &) {};
+    constexpr uint32_t kWalkBatch = P == P_MIXED ? 6u : 8u;
     if (staged)
-        walk_pipelined<D>(
+        walk_pipelined<D, kWalkBatch>(
             a.local_slice, nb, nc, pi, poly_in, x_per_r,
             [&](uint32_t r) { return Rec{load_pv<T, D>(s_pv, r), s_p[r]}; }, no_late, pair);
     else
-        walk_pipelined<D>(
+        walk_pipelined<D, kWalkBatch>(
             a.slice, nb, nc, pi, poly_in, x_per_r,
             [&](uint32_t j) { return Rec{load_pv<T, D>(a.posvol, j), a.p[j]}; }, no_late, pair);
     const T k_avg = T(2) * a.inv_rho0c0_sum * a.rho0c0; // 2 * AverageP / (p_i + p_j)
@@ -541,7 +566,7 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
 }
 
 template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
-__global__ void __launch_bounds__(kThreads) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 5 : 1))) k_pass1(FluidArgs<typename Prec<P>::T> a, double dt_d)
 {
     pass1_body<D, P, RIEMANN, FUSE_INIT_2ND, CORR>(a, xcd_block_index(), dt_d);
 }
@@ -553,6 +578,192 @@ template <class T>
 __device__ __forceinline__ T p_jump_scaled(const FluidArgs<T> &a, T u)
 {
     return a.riemann == SPHX_RIEMANN_ACOUSTIC ? u * dmin(T(3) * (a.inv_c0_ave * dmax(u, T(0))), T(1)) : u;
+}
+
+
+//---------------------------------------------------------------------------
+// Cell traversal of the inner neighbours, the alternative to stored lists
+// (kept for their A/B: sphx_acoustic_step_2nd_half_traversal).  One wave per
+// 64-particle chunk.  The cells of the chunk's search boxes (the bounding box
+// of its particles' cells grown by one cell each way) are staged in LDS one
+// (x, y) column of the box after another (a column's z cells are contiguous
+// in cell-list order) as the float record (x, y, z, Vol) and the particle
+// index.  Each lane then walks its own search box in the order of the stored
+// lists (the nine (x, y) columns x-major, each column's three z cells in
+// cell-list order) and takes a candidate by the test of the list build: the
+// float pre-filter with its rounding margin, the double criterion in the
+// margin shell, the particle itself excluded.  The neighbours and their order
+// are those of the lists, so the sums are the same bit for bit.  A chunk whose
+// box holds more than kTravCap particles or kTravCols columns reads its
+// candidates from global memory.  3D, FP32-mixed (the float record is the
+// pre-filter's position).
+//---------------------------------------------------------------------------
+struct TravArgs
+{
+    const uint32_t *cell_offset, *particle_index; // the fluid's cell linked list
+    MeshParams m;
+    PosPtrs pos;          // double positions, for the criterion in the margin shell
+    double inv_h, pre_xmax;
+    float thr_lo, thr_hi; // squared-distance bounds of the pre-filter
+};
+constexpr uint32_t kTravCols = 64;  // staged (x, y) columns per chunk at most
+constexpr uint32_t kTravCap = 1152; // staged candidates per chunk at most (20 B each)
+
+// NeighborCriterion in double, evaluated as the list build does (no contraction)
+__device__ __forceinline__ bool trav_exact(const TravArgs &t, const double *xi, uint32_t j)
+{
+#pragma clang fp contract(off)
+    const double s0 = t.inv_h * (xi[0] - t.pos.x[j]);
+    const double s1 = t.inv_h * (xi[1] - t.pos.y[j]);
+    const double s2 = t.inv_h * (xi[2] - t.pos.z[j]);
+    double r2 = s0 * s0 + s1 * s1;
+    r2 = r2 + s2 * s2;
+    return r2 < 4.0;
+}
+
+// a lane's place in the traversal: its cell and position, the chunk's box
+struct TravLane
+{
+    double xi[3];
+    int c[3];
+    int lo0, lo1, ny; // box origin (x, y) and its extent in y
+    bool staged;      // wave-uniform
+};
+
+// stage the chunk's box (every lane of the wave takes part)
+__device__ __forceinline__ TravLane trav_stage(const TravArgs &t, const Rec4<float> *posvol, uint32_t i, bool valid,
+                                               Rec4<float> *s_pv, uint32_t *s_j, uint32_t *col_src, uint32_t *col_dst)
+{
+    TravLane L{};
+    const uint32_t lane = threadIdx.x & 63u;
+    if (valid)
+    {
+        load_pos<3>(t.pos, i, L.xi);
+        cell_index<3>(t.m, L.xi, L.c);
+    }
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        lo[k] = max((int)wave_min_u32(valid ? (uint32_t)L.c[k] : ~0u) - 1, 0);
+        hi[k] = min((int)wave_max_u32(valid ? (uint32_t)L.c[k] : 0u) + 1, t.m.all_cells[k] - 1);
+    }
+    L.lo0 = lo[0];
+    L.lo1 = lo[1];
+    L.ny = hi[1] - lo[1] + 1;
+    const uint32_t ncol = (uint32_t)((hi[0] - lo[0] + 1) * L.ny);
+    uint32_t src = 0, len = 0;
+    if (lane < ncol)
+    {
+        const int l = lo[0] + (int)lane / L.ny, mm = lo[1] + (int)lane % L.ny;
+        const uint32_t first = (uint32_t)((l * t.m.all_cells[1] + mm) * t.m.all_cells[2] + lo[2]);
+        src = t.cell_offset[first];
+        len = t.cell_offset[first + (uint32_t)(hi[2] - lo[2]) + 1u] - src;
+    }
+    uint32_t incl = len; // inclusive scan over the lanes
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, off, 64);
+        if ((int)lane >= off)
+            incl += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+    L.staged = ncol <= kTravCols && total <= kTravCap;
+    if (L.staged)
+    {
+        if (lane < ncol)
+        {
+            col_src[lane] = src;
+            col_dst[lane] = incl - len;
+        }
+        __syncthreads();
+        for (uint32_t f = lane; f < total; f += 64u)
+        {
+            // the column of slot f: the last one starting at or before f
+            uint32_t a = 0, b = ncol;
+            while (b - a > 1u)
+            {
+                const uint32_t mid = (a + b) >> 1;
+                if (col_dst[mid] <= f)
+                    a = mid;
+                else
+                    b = mid;
+            }
+            const uint32_t j = t.particle_index[col_src[a] + (f - col_dst[a])];
+            s_pv[f] = posvol[j];
+            s_j[f] = j;
+        }
+    }
+    __syncthreads();
+    return L;
+}
+
+// walk the lane's search box; pair(j, record, geometry) for each neighbour j
+template <class Pair>
+__device__ __forceinline__ void trav_walk(const TravArgs &t, const TravLane &L, const Rec4<float> *posvol,
+                                          uint32_t i, const PV<float, 3> &pi, const PolyLDS<float> &lds,
+                                          float x_per_r, const Rec4<float> *s_pv, const uint32_t *s_j,
+                                          const uint32_t *col_src, const uint32_t *col_dst, Pair &&pair)
+{
+    constexpr float kFar = 1.0e30f;
+    bool exact_lane = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        exact_lane |= !(fabs(L.xi[k]) <= t.pre_xmax);
+    const float lo_thr = exact_lane ? -1.0f : t.thr_lo;
+    const float hi_thr = exact_lane ? kFar : t.thr_hi;
+    const int zlo = max(0, L.c[2] - 1), zhi = min(t.m.all_cells[2] - 1, L.c[2] + 1);
+    for (int r = 0; r < 9; ++r)
+    {
+        const int l = L.c[0] + r / 3 - 1, mm = L.c[1] + r % 3 - 1;
+        if (l < 0 || l >= t.m.all_cells[0] || mm < 0 || mm >= t.m.all_cells[1])
+            continue;
+        const uint32_t first = (uint32_t)((l * t.m.all_cells[1] + mm) * t.m.all_cells[2] + zlo);
+        const uint32_t s0 = t.cell_offset[first];
+        const uint32_t n = t.cell_offset[first + (uint32_t)(zhi - zlo) + 1u] - s0;
+        uint32_t base = 0;
+        if (L.staged)
+        {
+            const uint32_t q = (uint32_t)((l - L.lo0) * L.ny + (mm - L.lo1));
+            base = col_dst[q] + (s0 - col_src[q]);
+        }
+        for (uint32_t k = 0; k < n; ++k)
+        {
+            uint32_t j;
+            Rec4<float> p;
+            if (L.staged)
+            {
+                j = s_j[base + k];
+                p = s_pv[base + k];
+            }
+            else
+            {
+                j = t.particle_index[s0 + k];
+                p = posvol[j];
+            }
+            float fx = pi.x - p.x, fy = pi.y - p.y, fz = pi.z - p.z;
+            // the criterion's own copies: its products are not merged with the
+            // pair geometry's, which then contracts as in the list walk
+            __asm__ volatile("" : "+v"(fx), "+v"(fy), "+v"(fz));
+            float r2f = fx * fx;
+            r2f = fmaf(fy, fy, r2f);
+            r2f = fmaf(fz, fz, r2f);
+            bool in = j != i && r2f < lo_thr;
+            if (j != i && !in && r2f < hi_thr)
+                in = trav_exact(t, L.xi, j);
+            if (in)
+            {
+                const PV<float, 3> pj{p.x, p.y, p.z, p.w};
+                PairGeo<float> g;
+                int loc;
+                float f;
+                pair_geometry<3>(pi, pj, x_per_r, g, loc, f);
+                g.w = pair_kernel(*reinterpret_cast<const Coef4<float> *>(lds.c[loc]), f);
+                pair(j, pj, g);
+            }
+        }
+    }
 }
 
 //---------------------------------------------------------------------------
@@ -571,11 +782,15 @@ static_assert(kPass2Cap <= kStageCap, "pass 2 stages a prefix of the staging cap
 
 // FP32-mixed: registers capped for 4 waves per SIMD, which its 40 KB of LDS
 // per block allows; FP64 (not staged) keeps its registers
-template <int D, int P, bool RIEMANN, bool FUSE_CFL, bool CORR>
+// TRAV: the inner neighbours by the cell traversal (one-wave blocks, 3D,
+// FP32-mixed, not staged by the stored lists' sets)
+template <int D, int P, bool RIEMANN, bool FUSE_CFL, bool CORR, bool TRAV = false>
 __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> &a, const uint32_t blk, double dt_d,
-                                           unsigned long long *max_slot)
+                                           unsigned long long *max_slot, const TravArgs *trav = nullptr)
 {
     using T = typename Prec<P>::T;
+    static_assert(!TRAV || (D == 3 && P == P_MIXED && !CORR), "the traversal is 3D FP32-mixed, uncorrected");
+    constexpr uint32_t NT = TRAV ? 64u : kThreads; // threads per block
     if (!step_dt(a, dt_d)) // block-uniform: before any barrier
         return;
     __shared__ PolyLDS<T> poly_in;
@@ -585,7 +800,7 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
     stage_poly(poly_in, a.poly, true);
     for (int k = 0; k < a.n_wall; ++k)
         stage_poly(poly_w[k], a.wall[k].poly, true);
-    const uint32_t i = blk * kThreads + threadIdx.x;
+    const uint32_t i = blk * NT + threadIdx.x;
     // staged neighbour records: (x, y, z|Vol, Vol|0) and the velocity components
     Rec4<T> *s_pv = (Rec4<T> *)staged_lds;
     // (no opaque base as in pass 1: its extra register makes this pass spill,
@@ -607,6 +822,13 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
                 s_vz[r] = v.z;
         }
     }
+    // the traversal's staged box: records, particle indices, column starts
+    __shared__ uint32_t trav_cols[TRAV ? 2 * kTravCols : 1];
+    Rec4<float> *t_pv = (Rec4<float> *)staged_lds;
+    uint32_t *t_j = (uint32_t *)(t_pv + kTravCap);
+    TravLane tl{};
+    if constexpr (TRAV)
+        tl = trav_stage(*trav, a.posvol, i, i < a.n, t_pv, t_j, trav_cols, trav_cols + kTravCols);
     __syncthreads();
     double signal = -1.0;
     if (i < a.n)
@@ -658,7 +880,10 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
         // its pressures) makes this pass spill under its register cap
         auto no_late = [](uint32_t, Rec &) {};
         constexpr uint32_t kWalkBatch = P == P_MIXED ? 6u : 8u;
-        if (staged)
+        if constexpr (TRAV)
+            trav_walk(*trav, tl, a.posvol, i, pi, poly_in, x_per_r, t_pv, t_j, trav_cols, trav_cols + kTravCols,
+                      [&](uint32_t j, const PV<T, D> &pj, const PairGeo<T> &g) { pair(Rec{pj, a.vel[j]}, g); });
+        else if (staged)
             walk_pipelined<D, kWalkBatch>(
                 a.local_slice, nb, nc, pi, poly_in, x_per_r,
                 [&](uint32_t r) {
@@ -752,7 +977,7 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
         if (threadIdx.x == 0)
         {
             double b = wave_red[0];
-            for (int w = 1; w < kThreads / 64; ++w)
+            for (int w = 1; w < (int)(NT / 64); ++w)
                 b = fmax(b, wave_red[w]);
             atomic_max_spread(max_slot, b);
         }
@@ -764,6 +989,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P
 k_pass2(FluidArgs<typename Prec<P>::T> a, double dt_d, unsigned long long *max_slot)
 {
     pass2_body<D, P, RIEMANN, FUSE_CFL, CORR>(a, xcd_block_index(), dt_d, max_slot);
+}
+
+// pass 2 with the inner neighbours by the cell traversal: one 64-particle chunk per block
+template <bool RIEMANN>
+__global__ void __launch_bounds__(64) k_pass2_traverse(FluidArgs<float> a, TravArgs t, double dt_d)
+{
+    pass2_body<3, P_MIXED, RIEMANN, false, false, true>(a, xcd_block_index(), dt_d, nullptr, &t);
 }
 
 //---------------------------------------------------------------------------
@@ -1791,7 +2023,7 @@ __global__ void __launch_bounds__(kThreads) kb_init_1st(const SceneArgs<typename
 }
 
 template <int D, int P, bool RIEMANN>
-__global__ void __launch_bounds__(kThreads) kb_pass1(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 5 : 1))) kb_pass1(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
                                                      const uint32_t *__restrict__ map)
 {
     const uint32_t m = map[xcd_block_index()];
@@ -2166,7 +2398,7 @@ struct Acoustic1L
         }
         ProfileScope prof(b->ctx, "acoustic_pass1");
         const bool rs = riemann != SPHX_RIEMANN_NONE;
-        const size_t lds = a.stage_count ? (size_t)a.stage_cap * (sizeof(Rec4<T>) + sizeof(T)) : 0;
+        const size_t lds = a.stage_count ? (size_t)pass1_cap<P>() * (sizeof(Rec4<T>) + sizeof(T)) : 0;
         SPHX_PASS_VARIANTS(k_pass1, rs, fuse_init_2nd, corr, a, dt);
         LAUNCH_CHECK();
         return SPHX_OK;
@@ -2213,6 +2445,50 @@ sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int
     sphx_body *b = inner->source;
     unsigned long long *slot = fuse_cfl ? b->ctx->scalars_d + SLOT_MAX : nullptr;
     return dispatch<Acoustic2L>(b, inner, w, nw, riemann, dt, run_init, slot, false);
+}
+
+sphx_status acoustic_2nd_half_traversal(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann, double dt)
+{
+    sphx_body *b = inner->source;
+    if (b->D != 3 || b->P != P_MIXED)
+    {
+        set_error("the cell traversal runs 3D FP32-mixed bodies");
+        return SPHX_E_ARG;
+    }
+    if (!b->n)
+        return SPHX_OK;
+    const sphx_cell_linked_list *c = inner->cll;
+    FluidArgs<float> a = fluid_args<float>(b);
+    a.riemann = riemann;
+    attach_relations(a, inner, w, nw);
+    a.stage_count = nullptr; // the traversal stages its own boxes
+    TravArgs t;
+    t.cell_offset = c->cell_offset;
+    t.particle_index = c->particle_index;
+    t.m = c->mesh;
+    t.pos = PosPtrs{b->pos[0], b->pos[1], b->pos[2]};
+    t.inv_h = inner->inv_h;
+    relation_prefilter(inner, &t.thr_lo, &t.thr_hi, &t.pre_xmax);
+    {
+        ProfileScope prof(b->ctx, "acoustic_init");
+        hipLaunchKernelGGL((k_init_2nd<3, P_MIXED>), dim3(grid_for(b->n, kThreads)), dim3(kThreads), 0,
+                           b->ctx->stream, a, dt);
+    }
+    ProfileScope prof(b->ctx, "acoustic_pass2");
+    const size_t lds = (size_t)kTravCap * (sizeof(Rec4<float>) + sizeof(uint32_t));
+    const dim3 g(grid_for(b->n, 64)), blk(64);
+    if (riemann != SPHX_RIEMANN_NONE)
+    {
+        allow_lds(k_pass2_traverse<true>, lds);
+        hipLaunchKernelGGL(k_pass2_traverse<true>, g, blk, lds, b->ctx->stream, a, t, dt);
+    }
+    else
+    {
+        allow_lds(k_pass2_traverse<false>, lds);
+        hipLaunchKernelGGL(k_pass2_traverse<false>, g, blk, lds, b->ctx->stream, a, t, dt);
+    }
+    LAUNCH_CHECK();
+    return SPHX_OK;
 }
 
 sphx_status acoustic_1st_half_corrected(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
@@ -2626,7 +2902,10 @@ struct BatchL
     static const SceneArgs<T> *tab(const BatchFluid &b) { return (const SceneArgs<T> *)b.tab; }
     // dynamic LDS of the staged neighbour passes (as the single-scene launchers)
     static size_t lds_sum(const BatchFluid &b) { return (size_t)b.stage_cap * sizeof(Rec4<T>); }
-    static size_t lds_pass1(const BatchFluid &b) { return (size_t)b.stage_cap * (sizeof(Rec4<T>) + sizeof(T)); }
+    static size_t lds_pass1(const BatchFluid &b)
+    {
+        return b.stage_cap ? (size_t)pass1_cap<P>() * (sizeof(Rec4<T>) + sizeof(T)) : 0;
+    }
     static size_t lds_pass2(const BatchFluid &b)
     {
         return b.stage_cap ? (size_t)kPass2Cap * (sizeof(Rec4<T>) + 3 * sizeof(T)) : 0;

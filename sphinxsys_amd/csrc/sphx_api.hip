@@ -1362,6 +1362,21 @@ sphx_status sphx_acoustic_step_2nd_half(sphx_relation *inner, sphx_relation *con
     return acoustic_2nd_half(inner, walls, n, riemann, dt, true, false);
 }
 
+sphx_status sphx_acoustic_step_2nd_half_traversal(sphx_relation *inner, sphx_relation *const *walls, int n,
+                                                  int riemann, double dt)
+{
+    sphx_status s = check_fluid_relations(inner, walls, n, true);
+    if (s)
+        return s;
+    if (riemann < SPHX_RIEMANN_ACOUSTIC || riemann > SPHX_RIEMANN_DISSIPATIVE)
+        return arg_error("unknown Riemann solver");
+    const sphx_cell_linked_list *c = inner->cll;
+    if (!c->built || c->built_version != c->body->config_version)
+        return state_error("cell linked list not updated since the last configuration change");
+    SPHX_HIP(hipSetDevice(inner->source->ctx->device));
+    return acoustic_2nd_half_traversal(inner, walls, n, riemann, dt);
+}
+
 sphx_status sphx_acoustic_step_1st_half_corrected(sphx_relation *inner, sphx_relation *const *walls, int n,
                                                   int riemann, double dt)
 {

@@ -72,6 +72,40 @@ struct MeshParams
     int all_cells[3];
 };
 
+// Mesh::CellIndexFromPosition and the linear cell index, base_mesh.hxx:9-26,73-78
+template <int D>
+__device__ __forceinline__ void cell_index(const MeshParams &m, const double *x, int *c)
+{
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+    {
+        int ci = (int)floor((x[k] - m.lower[k]) / m.gs);
+        c[k] = min(max(ci, 0), m.all_grid_points[k] - 2);
+    }
+}
+template <int D>
+__device__ __forceinline__ uint32_t linear_cell(const MeshParams &m, const int *c)
+{
+    if (D == 2)
+        return (uint32_t)(c[0] * m.all_cells[1] + c[1]);
+    return (uint32_t)(c[0] * m.all_cells[1] * m.all_cells[2] + c[1] * m.all_cells[2] + c[2]);
+}
+
+// the double positions of a body (SoA)
+struct PosPtrs
+{
+    const double *x, *y, *z;
+};
+
+template <int D>
+__device__ __forceinline__ void load_pos(const PosPtrs &p, uint32_t i, double *x)
+{
+    x[0] = p.x[i];
+    x[1] = p.y[i];
+    if (D == 3)
+        x[2] = p.z[i];
+}
+
 // XCD-aware block index: gfx950 dispatches consecutive workgroups round-robin
 // over its 8 XCDs; remap so that each XCD works on one contiguous range of
 // particles and the neighbour gathers of a range stay in that XCD's L2.
@@ -118,6 +152,20 @@ __device__ __forceinline__ void atomic_max_spread(unsigned long long *base, doub
 }
 
 // wave64 reductions with DPP-free shuffles
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
 __device__ __forceinline__ double wave_max(double v)
 {
     for (int off = 32; off > 0; off >>= 1)

@@ -214,6 +214,10 @@ constexpr uint64_t kNoAllocMaxCells = 1ull << 22;
 sphx_status noalloc_scratch_init(int device);
 sphx_status cell_linked_list_update_noalloc(sphx_cell_linked_list *cll);
 sphx_status relation_to_csr(sphx_relation *rel);
+// squared-distance bounds of the neighbour build's float pre-filter (below
+// thr_lo a neighbour, at or above thr_hi not, the double criterion between)
+// and the coordinate bound |x| <= pre_xmax under which they hold
+void relation_prefilter(const sphx_relation *rel, float *thr_lo, float *thr_hi, double *pre_xmax);
 // scan = true forces the per-lane scan kernel (no LDS tile)
 sphx_status relation_build(sphx_relation *rel, uint32_t kmax, unsigned long long *needed, bool scan);
 // the LDS staging sets of a staged relation, from its finished slices
@@ -246,6 +250,9 @@ sphx_status substep_begin(sphx_context *ctx, double acoustic_cfl, double h);
 // fuse_cfl: max-reduce the next sub-step's signal speed into the SLOT_MAX lanes
 sphx_status acoustic_2nd_half(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                               double dt, bool run_init, bool fuse_cfl);
+// the same with the inner neighbours found by a cell traversal (3D FP32-mixed)
+sphx_status acoustic_2nd_half_traversal(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
+                                        double dt);
 // LinearCorrectionCK variants of the two passes (the 1st refreshes grad_sum when stale)
 sphx_status acoustic_1st_half_corrected(sphx_relation *inner, sphx_relation *const *w, int nw, int riemann,
                                         double dt, bool fuse_init_2nd);

@@ -51,6 +51,25 @@ def test_served_3d_stream_alone_and_in_the_batch(hip):
     assert server.tuning["steps"] > max(r.advection_steps for r in mixed)
 
 
+def test_served_3d_stream_20k_to_2m_to_t_end_1(hip):
+    """The served configuration at full size: four 3D dam breaks of 28k to 1.8M
+    fluid particles (log-uniform in 20k-2M) arriving at 4 requests/s, each run
+    to t = 1.0 s, served by continuous batching with two slots (later requests
+    wait for a slot to free); every scene's fields, step counts and physical
+    time are bit-identical to its run alone."""
+    specs = serving.served_3d_specs(count=4, rate=4.0, n_lo=20000, n_hi=2000000, t_end=1.0, seed=13)
+    alone, _ = serving.Server(hip, max_active=1).run(specs, keep_fields=True, realtime=False)
+    assert max(r.n_fluid for r in alone) > 1500000 and min(r.n_fluid for r in alone) < 50000
+    assert all(r.physical_time >= 1.0 for r in alone)
+    server = serving.BatchServer(hip, max_active=2)
+    mixed, _ = server.run(specs, keep_fields=True, realtime=True)
+    for a, b in zip(alone, mixed):
+        assert (a.acoustic_steps, a.advection_steps, a.physical_time) == \
+            (b.acoustic_steps, b.advection_steps, b.physical_time), a.ident
+        for k in a.fields:
+            assert np.array_equal(a.fields[k], b.fields[k]), (a.ident, k)
+
+
 def test_batched_2d_scenes_alone_and_in_the_batch(hip):
     specs = serving.batched_2d_specs(count=12, n_lo=1000, n_hi=20000, steps=25, seed=7)
     alone, _ = serving.Server(hip, max_active=1).run(specs, keep_fields=True, realtime=False)
