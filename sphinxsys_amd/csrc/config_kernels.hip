@@ -1138,13 +1138,28 @@ __device__ __forceinline__ void cll_clear_body(const CllScene &a, uint32_t c)
     if (c < a.ncells)
         a.cursor[c] = 0u;
 }
-// particle i at its cell's offset plus a slot taken atomically
+// particle i at its cell's offset plus a slot taken atomically: one atomic
+// per run of equal cells in the wave (particles are in cell order but for
+// those that moved since the last sort), the run's lanes in consecutive slots
+// from the run's base (every lane of the wave takes part)
 __device__ __forceinline__ void cll_scatter_body(const CllScene &a, uint32_t i)
 {
-    if (i >= a.n)
-        return;
-    const uint32_t key = a.keys[i];
-    a.particle_index[a.cell_offset[key] + atomicAdd(&a.cursor[key], 1u)] = i;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t key = i < a.n ? a.keys[i] : 0xffffffffu;
+    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1u, 64);
+    const bool head = lane == 0u || prev != key;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long above = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
+    const uint32_t next = above ? (uint32_t)__ffsll((long long)above) - 1u : 64u;
+    uint32_t base = 0u;
+    if (head && i < a.n)
+        base = atomicAdd(&a.cursor[key], next - lane);
+    // the run's head: the last head at or below this lane (lane 0 is one)
+    const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
+    const uint32_t lead = 63u - (uint32_t)__clzll((long long)(heads & upto));
+    base = (uint32_t)__shfl((int)base, (int)lead, 64);
+    if (i < a.n)
+        a.particle_index[a.cell_offset[key] + base + (lane - lead)] = i;
 }
 // the indices of cell c in ascending order (cells hold a few particles)
 __device__ __forceinline__ void cll_order_body(const CllScene &a, uint32_t c)
@@ -1153,6 +1168,26 @@ __device__ __forceinline__ void cll_order_body(const CllScene &a, uint32_t c)
         return;
     const uint32_t lo = a.cell_offset[c], hi = a.cell_offset[c + 1];
     uint32_t *idx = a.particle_index;
+    // most cells arrive in order (one run of the scatter): check that with
+    // independent loads, eight at a time, before sorting
+    bool sorted = true;
+    uint32_t last = 0u;
+    for (uint32_t q0 = lo; q0 < hi && sorted; q0 += 8u)
+    {
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k)
+            v[k] = q0 + k < hi ? idx[q0 + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k)
+            if (q0 + k < hi)
+            {
+                sorted = sorted && (q0 + k == lo || last < v[k]);
+                last = v[k];
+            }
+    }
+    if (sorted)
+        return;
     for (uint32_t q = lo + 1; q < hi; ++q)
     {
         const uint32_t v = idx[q];
