@@ -374,8 +374,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--profile-window", action="store_true",
-                    help="print host timestamps of the headline window of the FP32-mixed run (and of the "
-                         "copy calibration) for python -m sphinxsys_amd.profile_summary")
+                    help="print host timestamps of the headline window of the FP32-mixed run (of the serial "
+                         "and batched phases with --batched / --served, and of the copy calibration) for "
+                         "python -m sphinxsys_amd.profile_summary")
     ap.add_argument("--full-run", type=float, default=0.0, metavar="T_END",
                     help="instead of the throughput line: run the scene to T_END in FP32-mixed and FP64 "
                          "and compare probe pressure and surge front (one JSON line)")
@@ -386,6 +387,8 @@ def main():
     ap.add_argument("--served", type=int, default=0, metavar="N",
                     help="instead of the throughput line: a Poisson stream of N 3D dam-break requests "
                          "(20k-2M particles, t_end 1.0 s); p50/p99 time to solution")
+    ap.add_argument("--serial-only", action="store_true",
+                    help="with --batched / --served: only the serial plain runs (per-scene times)")
     ap.add_argument("--rate", type=float, default=4.0, help="arrival rate of --served (requests/s)")
     ap.add_argument("--slots", type=int, default=256, help="concurrent scenes of --batched / --served")
     ap.add_argument("--traversal-ab", action="store_true",
@@ -431,18 +434,30 @@ def main():
             f"({sum(sp.fluid_count() for sp in specs)} fluid particles)")
         arrivals = "arrivals in real time" if realtime else "all requests at t = 0"
         log(f"serial: one plain run after another, {arrivals}")
+        collect.resume("serial")
         res1, wall1 = serving.Server(lib, max_active=1).run(specs, realtime=realtime)
+        collect.pause()
         serial = serving.summarize(res1, wall1)
         log(f"  {serial['particle_steps_per_s']:.3g} particle-steps/s in {wall1:.1f}s, "
             f"p50 {serial['latency_p50_s']:.2f}s p99 {serial['latency_p99_s']:.2f}s")
+        per_scene = [{"n_fluid": r.n_fluid, "acoustic_steps": r.acoustic_steps, "serial_run_s": r.finish - r.start,
+                      "serial_rate": r.n_fluid * r.acoustic_steps / max(r.finish - r.start, 1e-9)} for r in res1]
+        if args.serial_only:
+            print(json.dumps({"metric": "serial plain runs of independent scenes", "workload": what,
+                              "serial": serial, "scenes": per_scene}), flush=True)
+            return
         log(f"batched: continuous batching, one graph launch per advection step, {arrivals}")
         server = serving.BatchServer(lib, max_active=args.slots)
+        collect.resume("batched")
         res2, wall2 = server.run(specs, realtime=realtime)
+        collect.pause()
         conc = serving.summarize(res2, wall2)
         log(f"  {conc['particle_steps_per_s']:.3g} particle-steps/s in {wall2:.1f}s, "
             f"p50 {conc['latency_p50_s']:.2f}s p99 {conc['latency_p99_s']:.2f}s")
         same = all(a.acoustic_steps == b.acoustic_steps and a.physical_time == b.physical_time
                    and a.advection_steps == b.advection_steps for a, b in zip(res1, res2))
+        for d, r in zip(per_scene, res2):
+            d["batched_latency_s"] = r.latency
         last_arrival = max(sp.arrival for sp in specs)
         print(json.dumps({"metric": "aggregate particle-steps/s and time to solution of independent scenes",
                           "workload": what, "slots": args.slots, "serial": serial, "batched": conc,
@@ -450,6 +465,7 @@ def main():
                           "last_arrival_s": last_arrival,
                           "arrival_bound": bool(realtime and last_arrival > 0.5 * wall2),
                           "same_steps_and_time_alone_and_in_mix": same,
+                          "scenes": per_scene,
                           "batch_tuning": server.tuning}), flush=True)
         return
     if args.traversal_ab:

@@ -90,6 +90,22 @@ def label_dispatches(rows, alternate):
     return labels
 
 
+def busy_us(rows):
+    """Time covered by at least one dispatch (the union of their intervals), us."""
+    spans = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows)
+    total, cur_s, cur_e = 0, None, None
+    for s0, e0 in spans:
+        if cur_e is None or s0 > cur_e:
+            if cur_e is not None:
+                total += cur_e - cur_s
+            cur_s, cur_e = s0, e0
+        else:
+            cur_e = max(cur_e, e0)
+    if cur_e is not None:
+        total += cur_e - cur_s
+    return total * 1e-3
+
+
 def trace_stats(trace_dir, span, alternate):
     rows = [r for r in csv.DictReader(open(find_csv(trace_dir, "kernel_trace.csv"))) if in_window(r, span)]
     if not rows:
@@ -98,7 +114,7 @@ def trace_stats(trace_dir, span, alternate):
     dur = collections.defaultdict(list)
     for r in rows:
         dur[labels[r["Dispatch_Id"]]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
-    return dur
+    return dur, busy_us(rows)
 
 
 def counter_means(pmc_dir, span, alternate):
@@ -147,7 +163,7 @@ def summarize(args):
     for a in args.alternate:
         k, v = a.split("=")
         alternate[k] = v.split(",")
-    dur = trace_stats(trace_dir, span, alternate)
+    dur, busy = trace_stats(trace_dir, span, alternate)
     pmc_runs = [run_arg(p) for p in args.pmc]
     counters = collections.defaultdict(dict)
     for d, sp in pmc_runs:
@@ -159,7 +175,8 @@ def summarize(args):
     out = []
     w = out.append
     w(f"Window `{args.window}`: {wall_us / 1e3:.1f} ms of host time, {total / 1e3:.1f} ms of kernel time "
-      f"({sum(len(v) for v in dur.values())} dispatches)\n")
+      f"({sum(len(v) for v in dur.values())} dispatches); the GPU runs at least one kernel "
+      f"{busy / 1e3:.1f} ms ({100 * busy / wall_us:.1f}% of the window)\n")
     w("| kernel | calls | avg us | min us | max us | share |")
     w("|---|---:|---:|---:|---:|---:|")
     order = sorted(dur, key=lambda k: -sum(dur[k]))

@@ -45,7 +45,8 @@ def test_window_selection_labels_and_traffic(tmp_path):
              "--alternate", "k_relation_tile=inner,contact", "--out", str(out)])
     text = out.read_text()
     # 6 dispatches in the window: pass 2 twice (1 us each), the tile build as inner (3 us) and contact (1 us)
-    assert "(6 dispatches)" in text
+    # busy: [2000, 12000] covered without gaps = 10 us of an 18.2 us window
+    assert "(6 dispatches); the GPU runs at least one kernel 0.0 ms (54.9% of the window)" in text
     assert "| `k_pass2<3, 1, true, true, false>` | 2 | 1.0 |" in text
     assert "| `k_relation_tile<3> [inner]` | 2 | 3.0 |" in text
     assert "| `k_relation_tile<3> [contact]` | 2 | 1.0 |" in text
