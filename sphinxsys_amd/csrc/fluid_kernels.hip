@@ -329,11 +329,7 @@ __device__ __forceinline__ void init_1st_body(const FluidArgs<typename Prec<P>::
     T rho = full_C<P>(c) * a.rho0;
     a.rho[i] = rho;
     a.p[i] = Prec<P>::c_minus_one ? a.p0 * c : a.p0 * (rho / a.rho0 - T(1));
-    Rec4<T> v = a.vel[i];
-    a.dpos[0][i] += (double)(v.x * dt * T(0.5));
-    a.dpos[1][i] += (double)(v.y * dt * T(0.5));
-    if (D == 3)
-        a.dpos[2][i] += (double)(v.z * dt * T(0.5));
+    // its displacement dpos += v dt / 2 is added by pass 1, which reads v anyway
 }
 
 template <int D, int P>
@@ -548,9 +544,15 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
     if (D == 3)
         a.force[2][i] = Fz;
     a.dC[i] = dC;
-    // UpdateKernel: v += (F_prior + F) / m * dt
+    // UpdateKernel: v += (F_prior + F) / m * dt; before it, the initialize's
+    // displacement dpos += v dt / 2 with the velocity it saw (nothing changes
+    // v between the two kernels) and, fused, the 2nd half's with the new one:
+    // the two additions of the separate kernels, in their order
     const T m_i = a.mass[i];
     Rec4<T> v = a.vel[i];
+    double d0 = a.dpos[0][i] + (double)(v.x * dt * T(0.5));
+    double d1 = a.dpos[1][i] + (double)(v.y * dt * T(0.5));
+    double d2 = D == 3 ? a.dpos[2][i] + (double)(v.z * dt * T(0.5)) : 0.0;
     v.x += (a.force_prior[0][i] + Fx) / m_i * dt;
     v.y += (a.force_prior[1][i] + Fy) / m_i * dt;
     if (D == 3)
@@ -558,11 +560,15 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
     a.vel[i] = v;
     if (FUSE_INIT_2ND)
     {
-        a.dpos[0][i] += (double)(v.x * dt * T(0.5));
-        a.dpos[1][i] += (double)(v.y * dt * T(0.5));
+        d0 += (double)(v.x * dt * T(0.5));
+        d1 += (double)(v.y * dt * T(0.5));
         if (D == 3)
-            a.dpos[2][i] += (double)(v.z * dt * T(0.5));
+            d2 += (double)(v.z * dt * T(0.5));
     }
+    a.dpos[0][i] = d0;
+    a.dpos[1][i] = d1;
+    if (D == 3)
+        a.dpos[2][i] = d2;
 }
 
 template <int D, int P, bool RIEMANN, bool FUSE_INIT_2ND, bool CORR>
