@@ -150,6 +150,22 @@ __device__ __forceinline__ typename Prec<P>::T full_C(typename Prec<P>::T stored
     return Prec<P>::c_minus_one ? stored + T(1) : stored;
 }
 
+// a b + c d as fma(a, b, c d), written out: left to the compiler, which of the
+// two products it fuses depends on the code around the sum, so the same pair
+// evaluated in two neighbour walks (staged records, global records) could
+// differ in the last bit; a sum of three is dot3(a, b, c, d, e, f) =
+// fma(e, f, fma(a, b, c d))
+__device__ __forceinline__ float mad2(float a, float b, float c, float d) { return fmaf(a, b, c * d); }
+__device__ __forceinline__ double mad2(double a, double b, double c, double d) { return fma(a, b, c * d); }
+__device__ __forceinline__ float dot3(float a, float b, float c, float d, float e, float f)
+{
+    return fmaf(e, f, mad2(a, b, c, d));
+}
+__device__ __forceinline__ double dot3(double a, double b, double c, double d, double e, double f)
+{
+    return fma(e, f, mad2(a, b, c, d));
+}
+
 // tabulated kernel polynomial in LDS
 template <class T>
 struct alignas(4 * sizeof(T)) PolyLDS
@@ -225,7 +241,7 @@ __device__ __forceinline__ void pair_geometry(const PV<T, D> &pi, const PV<T, D>
     g.dx = pi.x - pj.x;
     g.dy = pi.y - pj.y;
     g.dz = D == 3 ? pi.z - pj.z : T(0);
-    const T r2 = D == 3 ? g.dx * g.dx + g.dy * g.dy + g.dz * g.dz : g.dx * g.dx + g.dy * g.dy;
+    const T r2 = D == 3 ? dot3(g.dx, g.dx, g.dy, g.dy, g.dz, g.dz) : mad2(g.dx, g.dx, g.dy, g.dy);
     g.ir = inv_sqrt(r2);
     const T x = (r2 * g.ir) * x_per_r;
     loc = min((int)x, kPolyIntervals - 1);
@@ -373,14 +389,14 @@ struct Mat3
     {
         if (D == 3)
         {
-            ox = m[0][0] * x + m[0][1] * y + m[0][2] * z;
-            oy = m[1][0] * x + m[1][1] * y + m[1][2] * z;
-            oz = m[2][0] * x + m[2][1] * y + m[2][2] * z;
+            ox = dot3(m[0][0], x, m[0][1], y, m[0][2], z);
+            oy = dot3(m[1][0], x, m[1][1], y, m[1][2], z);
+            oz = dot3(m[2][0], x, m[2][1], y, m[2][2], z);
         }
         else
         {
-            ox = m[0][0] * x + m[0][1] * y;
-            oy = m[1][0] * x + m[1][1] * y;
+            ox = mad2(m[0][0], x, m[0][1], y);
+            oy = mad2(m[1][0], x, m[1][1], y);
             oz = T(0);
         }
     }
@@ -858,16 +874,16 @@ __device__ __forceinline__ void pass2_body(const FluidArgs<typename Prec<P>::T> 
             const T dWV = g.w * rj.pv.vol;
             const T ir = g.ir;
             const T ex = g.dx * ir, ey = g.dy * ir, ez = D == 3 ? g.dz * ir : T(0);
-            T u = (vi.x - vj.x) * ex + (vi.y - vj.y) * ey;
+            T u = mad2(vi.x - vj.x, ex, vi.y - vj.y, ey);
             if (D == 3)
-                u += (vi.z - vj.z) * ez;
+                u = fma(vi.z - vj.z, ez, u);
             if (CORR)
             {
                 T bx, by, bz;
                 Bi.apply(ex, ey, ez, bx, by, bz);
-                T uc = (vi.x - vj.x) * bx + (vi.y - vj.y) * by;
+                T uc = mad2(vi.x - vj.x, bx, vi.y - vj.y, by);
                 if (D == 3)
-                    uc += (vi.z - vj.z) * bz;
+                    uc = fma(vi.z - vj.z, bz, uc);
                 div += uc * dWV;
             }
             else
