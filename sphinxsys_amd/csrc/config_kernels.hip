@@ -804,11 +804,13 @@ __device__ __forceinline__ void relation_tile_body(const BuildArgs &b, const uin
         // bounds the rounding of any evaluation order
         const float lo_thr = s_exact ? -1.0f : a.thr_lo;
         const float hi_thr = s_exact ? kFarAway : a.thr_hi;
-        uint32_t k = 0; // neighbours of source col (wave-uniform)
         if (u_total <= 64u * QMAX)
         {
             // every batch of the cached candidates at once: neighbour and
-            // margin-shell lane masks
+            // margin-shell lane masks (the scalar unit, shared by the CU's
+            // SIMDs, is the busier one here: the source itself is moved out of
+            // range by a select instead of a third mask, and the neighbours
+            // are counted from the stored masks at the end)
             unsigned long long hit[QMAX], shell[QMAX], any_shell = 0;
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
@@ -818,14 +820,10 @@ __device__ __forceinline__ void relation_tile_body(const BuildArgs &b, const uin
                 r2f = fmaf(fy, fy, r2f);
                 if (D == 3)
                     r2f = fmaf(fz, fz, r2f);
-                // one compare per mask, combined as scalar masks
-                const unsigned long long other = lane_mask(cj[q] != self);
-                const unsigned long long lo = lane_mask(r2f < lo_thr);
-                const unsigned long long hi = lane_mask(r2f < hi_thr);
-                hit[q] = other & lo;
-                shell[q] = other & hi & ~lo;
+                r2f = cj[q] == self ? kFarAway : r2f; // never below either bound
+                hit[q] = lane_mask(r2f < lo_thr);
+                shell[q] = lane_mask(r2f < hi_thr) & ~hit[q];
                 any_shell |= shell[q];
-                k += (uint32_t)__popcll(hit[q]);
             }
             if (any_shell)
             {
@@ -837,16 +835,17 @@ __device__ __forceinline__ void relation_tile_body(const BuildArgs &b, const uin
                         bool in = false;
                         if (lane_in(shell[q], lane))
                             in = exact(cj[q]);
-                        const unsigned long long add = lane_mask(in); // shell bits are not in hit
-                        hit[q] |= add;
-                        k += (uint32_t)__popcll(add);
+                        hit[q] |= lane_mask(in); // shell bits are not in hit
                     }
             }
+            // lane q stores mask q
             const uint32_t nq = (u_total + 63u) >> 6;
+            unsigned long long m = hit[0];
 #pragma unroll
-            for (int q = 0; q < QMAX; ++q)
-                if ((uint32_t)q < nq && lane == 0)
-                    my_masks[col][q] = hit[q];
+            for (int q = 1; q < QMAX; ++q)
+                m = lane == (uint32_t)q ? hit[q] : m;
+            if (lane < nq)
+                my_masks[col][lane] = m;
         }
         else
         {
@@ -874,12 +873,13 @@ __device__ __forceinline__ void relation_tile_body(const BuildArgs &b, const uin
                 const unsigned long long m = lane_mask(in);
                 if (lane == 0)
                     my_masks[col][f0 >> 6] = m;
-                k += (uint32_t)__popcll(m);
             }
         }
-        if (lane == col)
-            my_count = k;
     }
+    // each lane counts its own neighbours from its masks (a source's
+    // candidate list is its cell's: my_total entries)
+    for (uint32_t q = 0; q < (my_total + 63u) >> 6; ++q)
+        my_count += (uint32_t)__popcll(my_masks[lane][q]);
     const uint32_t mine = valid ? min(my_count, kmax) : 0u;
     if (valid)
     {
