@@ -745,20 +745,20 @@ __device__ __forceinline__ void relation_tile_body(const BuildArgs &b, const uin
                 u_total += lane_u32(len[r], lead);
             }
             const uint32_t fill = u_total <= 64u * QMAX ? u_total : 0u;
-            // slot f = 64 q + lane of the flat list: row by row, only the
-            // (row, batch) pairs that overlap (wave-uniform tests) do work
+            // slot f = 64 q + lane of the flat list: its row is the last one
+            // starting at or before f (an empty row starts where the next
+            // does), picked by vector selects over the uniform row starts; the
+            // shift from a slot to its particle_index position is mod 2^32
 #pragma unroll
-            for (int r = 0; r < R; ++r)
+            for (int q = 0; q < QMAX; ++q)
             {
-                const uint32_t a0 = u_p[r], a1 = r + 1 < R ? u_p[r + 1] : fill;
+                const uint32_t f = 64u * q + lane;
+                uint32_t shift = u_s0[0];
 #pragma unroll
-                for (int q = 0; q < QMAX; ++q)
-                    if (a0 < a1 && 64u * q < a1 && 64u * q + 64u > a0)
-                    {
-                        const uint32_t f = 64u * q + lane;
-                        if (f >= a0 && f < a1)
-                            cj[q] = a.particle_index[u_s0[r] + (f - a0)];
-                    }
+                for (int r = 1; r < R; ++r)
+                    shift = f >= u_p[r] ? u_s0[r] - u_p[r] : shift;
+                if (f < fill)
+                    cj[q] = a.particle_index[f + shift];
             }
 #pragma unroll
             for (int q = 0; q < QMAX; ++q)
