@@ -1162,12 +1162,58 @@ __device__ __forceinline__ void cll_scatter_body(const CllScene &a, uint32_t i)
         a.particle_index[a.cell_offset[key] + base + (lane - lead)] = i;
 }
 // the indices of cell c in ascending order (cells hold a few particles)
+// ascending order of N values in registers (bitonic network, N a power of 2)
+template <int N>
+__device__ __forceinline__ void sort_network(uint32_t (&v)[N])
+{
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+            {
+                const int l = i ^ j;
+                if (l > i)
+                {
+                    const uint32_t x = v[i], y = v[l];
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? min(x, y) : max(x, y);
+                    v[l] = up ? max(x, y) : min(x, y);
+                }
+            }
+}
+
+constexpr uint32_t kCellSortRegs = 32; // cells up to this size are ordered in registers
+
 __device__ __forceinline__ void cll_order_body(const CllScene &a, uint32_t c)
 {
     if (c >= a.ncells)
         return;
     const uint32_t lo = a.cell_offset[c], hi = a.cell_offset[c + 1];
     uint32_t *idx = a.particle_index;
+    if (hi - lo <= kCellSortRegs)
+    {
+        // the cell in registers (past its end the largest value): check, and
+        // sort only a cell whose particles arrived in several runs
+        const uint32_t n = hi - lo;
+        uint32_t v[kCellSortRegs];
+#pragma unroll
+        for (uint32_t k = 0; k < kCellSortRegs; ++k)
+            v[k] = k < n ? idx[lo + k] : 0xffffffffu;
+        bool sorted = true;
+#pragma unroll
+        for (uint32_t k = 1; k < kCellSortRegs; ++k)
+            sorted = sorted && (k >= n || v[k - 1] < v[k]);
+        if (sorted)
+            return;
+        sort_network(v);
+#pragma unroll
+        for (uint32_t k = 0; k < kCellSortRegs; ++k)
+            if (k < n)
+                idx[lo + k] = v[k];
+        return;
+    }
     // most cells arrive in order (one run of the scatter): check that with
     // independent loads, eight at a time, before sorting
     bool sorted = true;
