@@ -95,6 +95,43 @@ def test_cell_linked_list_bit_exact(hip, oracle, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_cell_linked_list_dense_cells_bit_exact(hip, oracle, dim):
+    """Cells of 1 to 150 particles, shuffled so that each cell's particles
+    arrive in many runs of the scatter: the per-cell ordering in registers (up
+    to 32 particles), in memory above that, and the 32/33 boundary, against the
+    oracle."""
+    rng = np.random.default_rng(300 + dim)
+    dp = 0.01
+    probe_ctx = ck.Context(hip)
+    probe = ck.Body(probe_ctx, dim, 1, dp)
+    mesh = ck.CellLinkedList(probe, np.zeros(dim), np.ones(dim))
+    lower, gs, cells = np.array(mesh.lower), mesh.grid_spacing, np.array(mesh.all_cells)
+    mesh.close(), probe.close(), probe_ctx.close()
+    sizes = [1, 2, 5, 31, 32, 33, 34, 64, 100, 150] * 3
+    # distinct cells two apart along every axis, well inside the mesh
+    free = np.stack(np.meshgrid(*[np.arange(3, c - 3, 2) for c in cells], indexing="ij"), -1).reshape(-1, dim)
+    picked = free[rng.choice(len(free), size=len(sizes), replace=False)]
+    pts = [lower + (c + 0.5 + rng.uniform(-0.4, 0.4, size=(k, dim))) * gs for c, k in zip(picked, sizes)]
+    pos = np.concatenate(pts)[rng.permutation(sum(sizes))]
+    out = []
+    for lib in (hip, oracle):
+        ctx = ck.Context(lib)
+        b = ck.Body(ctx, dim, len(pos), dp)
+        b.write("Position", pos)
+        c = ck.CellLinkedList(b, np.zeros(dim), np.ones(dim))
+        ck.UpdateCellLinkedList(c).exec()
+        out.append(c.read())
+        c.close(), b.close(), ctx.close()
+    (go, gi), (co, ci) = out
+    counts = np.diff(go.astype(np.int64))
+    assert sorted(counts[counts > 0].tolist()) == sorted(sizes)
+    assert np.array_equal(go, co)
+    assert np.array_equal(gi, ci)
+    for k in np.flatnonzero(counts > 1):
+        assert np.all(np.diff(gi[go[k]:go[k + 1]].astype(np.int64)) > 0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_particle_sort_bit_exact(hip, oracle, dim):
     rng = np.random.default_rng(200 + dim)
     pos = rng.uniform(0.0, 1.0, size=(30000, dim))
