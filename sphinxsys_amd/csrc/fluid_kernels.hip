@@ -555,10 +555,16 @@ __device__ __forceinline__ void pass1_body(const FluidArgs<typename Prec<P>::T> 
                 dC += wcd * a.inv_rho0c0_ave * C_i;
         }
     }
-    a.force[0][i] = Fx;
-    a.force[1][i] = Fy;
-    if (D == 3)
-        a.force[2][i] = Fz;
+    // inside the device-side sub-step loop the 2nd half, which sets the force
+    // anew, runs next whenever this pass runs and nothing reads the force in
+    // between: the store is left out there
+    if (!a.step)
+    {
+        a.force[0][i] = Fx;
+        a.force[1][i] = Fy;
+        if (D == 3)
+            a.force[2][i] = Fz;
+    }
     a.dC[i] = dC;
     // UpdateKernel: v += (F_prior + F) / m * dt; before it, the initialize's
     // displacement dpos += v dt / 2 with the velocity it saw (nothing changes
