@@ -1098,8 +1098,10 @@ __device__ __forceinline__ void summation_body(const FluidArgs<typename Prec<P>:
     a.S[i] = s;
 }
 
+// FP32-mixed: six blocks a CU (26 KB of staged records each, 66 VGPRs)
 template <int D, int P>
-__global__ void __launch_bounds__(kThreads) k_summation(FluidArgs<typename Prec<P>::T> a)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 6 : 1)))
+k_summation(FluidArgs<typename Prec<P>::T> a)
 {
     summation_body<D, P>(a, xcd_block_index());
 }
@@ -1978,8 +1980,8 @@ __device__ __forceinline__ uint32_t map_scene(uint32_t m) { return m >> 16; }
 __device__ __forceinline__ uint32_t map_block(uint32_t m) { return m & 0xffffu; }
 
 template <int D, int P>
-__global__ void __launch_bounds__(kThreads) kb_summation(const SceneArgs<typename Prec<P>::T> *__restrict__ tab,
-                                                         const uint32_t *__restrict__ map)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P == P_MIXED ? 6 : 1)))
+kb_summation(const SceneArgs<typename Prec<P>::T> *__restrict__ tab, const uint32_t *__restrict__ map)
 {
     const uint32_t m = map[xcd_block_index()];
     summation_body<D, P>(tab[map_scene(m)].f, map_block(m));
